@@ -1,0 +1,161 @@
+/*
+ * spidjoin.h -- C ABI of libspidjoin.so (MI355X / gfx950).
+ *
+ * The drop-in boundary of the reference is the C++ class
+ * pidjoin::JoinInstance (include/join_internals.hpp:301-323 of the reference,
+ * implemented in src/query.cc).  The reference has no C or FFI layer; this
+ * header is the plain-C form of that same surface (plain pointers and sizes,
+ * no C++ or torch types), on which this library's own C++ JoinInstance
+ * (pidjoin.hpp) is built and which ctypes / cgo / JNI stubs can bind.
+ *
+ * Conventions
+ *   - Tuples are 8-byte {uint32 key; uint32 rid} (kv_pair_t,
+ *     type_defs.hpp:53-56 of the reference).  Key 0 means "empty" and never
+ *     joins (glb_partition_count.c:192).  R keys must be unique (PK side).
+ *   - Output pairs are {uint32 r_rid; uint32 s_rid}, one per matching S tuple
+ *     (a2a_srj_probe_inner_mt.c:142-143), in no particular order, with an
+ *     exact count (no zero padding).
+ *   - Every function returns 0 on success or a negative SPJ_E* code; the
+ *     message of the last error on this thread is spj_last_error().  Nothing
+ *     calls exit().  (The reference printf()s and exit(-1)s instead.)
+ */
+#ifndef SPIDJOIN_H
+#define SPIDJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint32_t key; uint32_t rid; } spj_tuple_t;
+typedef struct { uint32_t r_rid; uint32_t s_rid; } spj_pair_t;
+
+enum {
+    SPJ_OK = 0,
+    SPJ_EINVAL = -1,       /* bad argument */
+    SPJ_EHIP = -2,         /* HIP runtime error */
+    SPJ_ENOMEM = -3,       /* device or host allocation failed */
+    SPJ_EDUPKEY = -4,      /* duplicate R key (a2a_srj_build.c:265-271) */
+    SPJ_EOVERFLOW = -5,    /* LDS table or output capacity exceeded */
+    SPJ_ESTATE = -6,       /* call out of order (e.g. FinishJoin before StartJoin) */
+    SPJ_EBUSY = -7         /* admission refused (serving HBM budget) */
+};
+
+/* Join algorithm (JOIN_DESIGN_JOIN_ALGORITHM_*, idpHandler.hpp:126-129). */
+enum { SPJ_ALG_AUTO = -1, SPJ_ALG_PHJ = 0, SPJ_ALG_NPHJ = 1, SPJ_ALG_SMJ = 2, SPJ_ALG_NL = 3 };
+
+/* Tunables; -1 = automatic (tuner table, else built-in defaults). */
+typedef struct {
+    int32_t algorithm;        /* SPJ_ALG_*: AUTO = hash, nested loop for tiny partitions */
+    int32_t radix_bits;       /* partition-pass radix bits (0..12) */
+    int32_t table_log2;       /* LDS hash-table slots, log2 (6..14) */
+    int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
+    int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
+    int32_t scatter_ipt;      /* scatter tuples per thread per round: 8 or 16 */
+    int64_t tile_tuples;      /* tuples per partition-pass tile */
+    int64_t items_s_max;      /* S tuples per join work item */
+} spj_config_t;
+
+typedef struct {
+    float heavy_ms, hist_ms, scan_ms, scatter_ms, items_ms, join_ms, total_ms;
+} spj_phase_times_t;
+
+void spj_config_default(spj_config_t *cfg);
+const char *spj_last_error(void);
+
+/* Device selection and facts (hipGetDeviceProperties). */
+int spj_init(int device);
+int spj_device_info(int *cus, uint64_t *lds_per_block, char *arch, int arch_len);
+
+/* Device memory helpers, so hosts without a GPU framework can drive the
+ * library (the reference's SEND / RECV host<->device moves,
+ * queryHostFunc.cc:1058-1126, 1421-1498). */
+int spj_dev_alloc(void **ptr, uint64_t bytes);
+int spj_dev_free(void *ptr);
+int spj_h2d(void *dst, const void *src, uint64_t bytes);
+int spj_d2h(void *dst, const void *src, uint64_t bytes);
+int spj_dev_memset(void *ptr, int value, uint64_t bytes);
+int spj_dev_sync(void);
+
+/* One join on device-resident relations: the whole plan of
+ * execution.hpp:15-157 (partition R, partition S, join) on one stream.
+ * out must hold out_cap >= nS pairs.  times may be NULL. */
+int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                    uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg, spj_phase_times_t *times);
+
+/* Same on host buffers (H2D, join, D2H of exactly out_count pairs). */
+int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                  uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg);
+
+/* Many independent joins in one launch sequence and one persistent join
+ * kernel fed by a device work queue.  Arrays have n entries; out[i] holds
+ * nS[i] pairs; out_count[i] receives instance i's exact count.  cfg applies
+ * to all instances (per-instance parameters come from the tuner when the
+ * fields are -1).  No counterpart in the reference (one join per
+ * JoinInstance). */
+int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR, const spj_tuple_t *const *S,
+                          const uint64_t *nS, spj_pair_t *const *out, uint64_t *out_count, const spj_config_t *cfg,
+                          spj_phase_times_t *times);
+
+/* ---- reference-shaped instance API (pidjoin::JoinInstance, query.cc) ---- */
+typedef struct spj_instance spj_instance_t;
+
+/* join_design_t (idpHandler.hpp:131-145); fields without a GPU meaning
+ * (rank_num, zipf_factor, rs_ratio, input_size) are accepted and ignored. */
+typedef struct {
+    int64_t input_size;
+    int32_t rank_num;
+    int32_t zipf_factor;
+    int32_t rs_ratio;
+    int32_t rank_set;   /* > 1: rank-set replication -> heavy-key handling on */
+    int32_t bank_set;   /* > 1: bank-set replication -> heavy-key handling on */
+    int32_t join_alg;   /* 0 phj, 1 nphj (served by phj), 2 smj, 3 nl */
+} spj_join_design_t;
+
+void spj_join_design_default(spj_join_design_t *d);   /* idpHandler.hpp:133-144 defaults */
+
+/* JoinInstance(num_rank_allocated, num_rankgroup, num_rank_in_rankgroup), query.cc:842 */
+spj_instance_t *spj_create(int num_rank_allocated, int num_rankgroup, int num_rank_in_rankgroup);
+/* LoadColumn(data, n, "left" | "right"), query.cc:213: the caller keeps data alive until spj_finish. */
+int spj_load_column(spj_instance_t *inst, const void *data, int64_t num_tuples, const char *col_name);
+/* PrepareJoin, query.cc:178: allocate device buffers. */
+int spj_prepare(spj_instance_t *inst, const spj_join_design_t *design);
+/* StartJoin, query.cc:188: asynchronous -- enqueue H2D + kernels and return. */
+int spj_start(spj_instance_t *inst, const spj_join_design_t *design);
+/* FinishJoin, query.cc:193: wait, copy exactly *count pairs into a library-
+ * allocated host buffer the caller frees with spj_free_result. */
+int spj_finish(spj_instance_t *inst, spj_pair_t **pairs, uint64_t *count);
+/* ExecuteJoin("phj" | "smj", design), query.cc:199: start + finish. */
+int spj_execute(spj_instance_t *inst, const char *join_type, const spj_join_design_t *design, spj_pair_t **pairs,
+                uint64_t *count);
+int spj_last_phase_times(spj_instance_t *inst, spj_phase_times_t *times);
+void spj_free_result(spj_pair_t *pairs);
+void spj_destroy(spj_instance_t *inst);
+
+/* ---- autotuner: per size / skew bucket parameter table ---- */
+/* Set / get the parameters of the bucket (ceil log2 nR, ceil log2 nS/nR,
+ * skew class of zipf) -- used by every later call whose config fields are -1. */
+int spj_tuner_set(uint64_t nR, uint64_t nS, double zipf, const spj_config_t *cfg);
+int spj_tuner_get(uint64_t nR, uint64_t nS, double zipf, spj_config_t *cfg);
+int spj_tuner_clear(void);
+int spj_tuner_entries(void);
+int spj_tuner_save(const char *path);
+int spj_tuner_load(const char *path);
+/* Time candidate radix bits / items-per-thread / LDS table sizes on these
+ * device relations and store the fastest for their bucket. */
+int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out, double zipf,
+                 int reps, float *best_ms);
+
+/* ---- synthetic workloads (seeded; identical bytes for a given seed) ---- */
+/* R: keys = random permutation of 1..nR, rid = row. */
+int spj_gen_r(spj_tuple_t *R, uint64_t nR, uint64_t seed);
+/* S: keys drawn Zipf(zipf) over R's keys in a seeded random rank order
+ * (zipf = 0: uniform), rid = row.  threads <= 0: all cores. */
+int spj_gen_s(spj_tuple_t *S, uint64_t nS, const spj_tuple_t *R, uint64_t nR, double zipf, uint64_t seed,
+              int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,280 @@
+// spj_capi.cpp -- extern "C" surface of libspidjoin.so (include/spidjoin.h).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "spidjoin.h"
+#include "spj_capi_internal.hpp"
+#include "spj_engine.hpp"
+#include "spj_tuner.hpp"
+
+namespace spj {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory) return fail(SPJ_ENOMEM, "%s: %s", what, hipGetErrorString(e));
+    return fail(SPJ_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int err_from_flags(uint32_t f, size_t inst)
+{
+    if (f & kErrDuplicateKey) return fail(SPJ_EDUPKEY, "instance %zu: duplicate key on the R (build) side", inst);
+    if (f & kErrTableOverflow) return fail(SPJ_EOVERFLOW, "instance %zu: a partition overflowed its LDS hash table", inst);
+    if (f & kErrOutOverflow) return fail(SPJ_EOVERFLOW, "instance %zu: output capacity exceeded", inst);
+    return SPJ_OK;
+}
+
+Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_hint)
+{
+    Resolved r = tuned_params((uint32_t)nR, (uint32_t)nS, skew_hint);
+    if (!cfg) return r;
+    if (cfg->algorithm >= 0) r.alg = (cfg->algorithm == SPJ_ALG_NPHJ) ? 0u : (uint32_t)cfg->algorithm;
+    if (cfg->radix_bits >= 0) r.bits = (uint32_t)cfg->radix_bits;
+    if (cfg->table_log2 >= 0) r.table_log2 = (uint32_t)cfg->table_log2;
+    if (cfg->heavy >= 0) r.heavy = (uint32_t)cfg->heavy;
+    if (cfg->heavy_min_count >= 0) r.heavy_min = (uint32_t)cfg->heavy_min_count;
+    if (cfg->scatter_ipt >= 0) r.ipt = (uint32_t)cfg->scatter_ipt;
+    if (cfg->tile_tuples > 0) r.tile = (uint32_t)cfg->tile_tuples;
+    if (cfg->items_s_max > 0) r.smax = (uint32_t)cfg->items_s_max;
+    return r;
+}
+
+int check_resolved(const Resolved &r)
+{
+    if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
+    if (r.table_log2 < 6 || r.table_log2 > 14) return fail(SPJ_EINVAL, "table_log2 %u outside 6..14", r.table_log2);
+    if (!(r.alg == 0 || r.alg == 2 || r.alg == 3)) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
+    if (!(r.ipt == 8 || r.ipt == 16)) return fail(SPJ_EINVAL, "scatter_ipt %u (8 or 16)", r.ipt);
+    if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);
+    if (r.smax < 512) return fail(SPJ_EINVAL, "items_s_max %u < 512", r.smax);
+    return SPJ_OK;
+}
+
+static std::mutex g_ctx_mu;
+static Context *g_ctx = nullptr;
+
+Context &default_context()
+{
+    if (!g_ctx) g_ctx = new Context();
+    return *g_ctx;
+}
+
+std::mutex &default_context_mutex() { return g_ctx_mu; }
+
+void fill_times(const PhaseTimes &t, spj_phase_times_t *o)
+{
+    if (!o) return;
+    o->heavy_ms = t.heavy;
+    o->hist_ms = t.hist;
+    o->scan_ms = t.scan;
+    o->scatter_ms = t.scatter;
+    o->items_ms = t.items;
+    o->join_ms = t.join;
+    o->total_ms = t.total;
+}
+
+int validate_sizes(uint64_t nR, uint64_t nS)
+{
+    if (nR >= 0xFFFFFFFFull || nS >= 0xFFFFFFFFull)
+        return fail(SPJ_EINVAL, "relation sizes must be below 2^32 - 1 tuples (got %llu, %llu)",
+                    (unsigned long long)nR, (unsigned long long)nS);
+    return SPJ_OK;
+}
+
+}  // namespace spj
+
+using namespace spj;
+
+extern "C" {
+
+void spj_config_default(spj_config_t *cfg)
+{
+    if (!cfg) return;
+    cfg->algorithm = SPJ_ALG_AUTO;
+    cfg->radix_bits = -1;
+    cfg->table_log2 = -1;
+    cfg->heavy = -1;
+    cfg->heavy_min_count = -1;
+    cfg->scatter_ipt = -1;
+    cfg->tile_tuples = -1;
+    cfg->items_s_max = -1;
+}
+
+const char *spj_last_error(void) { return g_err.c_str(); }
+
+int spj_init(int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= n) return fail(SPJ_EINVAL, "device %d of %d", device, n);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    (void)device_info();
+    return SPJ_OK;
+}
+
+int spj_device_info(int *cus, uint64_t *lds, char *arch, int arch_len)
+{
+    const DeviceInfo &d = device_info();
+    if (d.cus == 0) return fail(SPJ_EHIP, "no HIP device");
+    if (cus) *cus = d.cus;
+    if (lds) *lds = d.lds_per_block;
+    if (arch && arch_len > 0) {
+        strncpy(arch, d.arch.c_str(), (size_t)arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return SPJ_OK;
+}
+
+int spj_dev_alloc(void **p, uint64_t bytes)
+{
+    if (!p) return fail(SPJ_EINVAL, "null pointer");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMalloc");
+}
+
+int spj_dev_free(void *p)
+{
+    hipError_t e = hipFree(p);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipFree");
+}
+
+int spj_h2d(void *dst, const void *src, uint64_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemcpy H2D");
+}
+
+int spj_d2h(void *dst, const void *src, uint64_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemcpy D2H");
+}
+
+int spj_dev_memset(void *p, int v, uint64_t bytes)
+{
+    hipError_t e = hipMemset(p, v, bytes);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemset");
+}
+
+int spj_dev_sync(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
+int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                    uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg, spj_phase_times_t *times)
+{
+    if ((!R && nR) || (!S && nS) || (!out && nS) || !out_count) return fail(SPJ_EINVAL, "null argument");
+    if (out_cap < nS) return fail(SPJ_EINVAL, "out_cap %llu < |S| %llu", (unsigned long long)out_cap, (unsigned long long)nS);
+    int rc = validate_sizes(nR, nS);
+    if (rc) return rc;
+    InstSpec s;
+    s.R = reinterpret_cast<const uint2 *>(R);
+    s.S = reinterpret_cast<const uint2 *>(S);
+    s.out = reinterpret_cast<uint2 *>(out);
+    s.nR = (uint32_t)nR;
+    s.nS = (uint32_t)nS;
+    s.cfg = resolve(cfg, nR, nS, -1.0);
+    rc = check_resolved(s.cfg);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(default_context_mutex());
+    std::vector<uint32_t> counts, errs;
+    PhaseTimes t;
+    hipError_t e = default_context().run({s}, counts, errs, times ? &t : nullptr);
+    if (e != hipSuccess) return hip_fail(e, "join");
+    fill_times(t, times);
+    *out_count = counts[0];
+    return err_from_flags(errs[0], 0);
+}
+
+int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                  uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg)
+{
+    if ((!R && nR) || (!S && nS) || (!out && nS) || !out_count) return fail(SPJ_EINVAL, "null argument");
+    int rc = validate_sizes(nR, nS);
+    if (rc) return rc;
+    void *dR = nullptr, *dS = nullptr, *dO = nullptr;
+    hipError_t e = hipMalloc(&dR, nR * 8 + 8);
+    if (e == hipSuccess) e = hipMalloc(&dS, nS * 8 + 8);
+    if (e == hipSuccess) e = hipMalloc(&dO, nS * 8 + 8);
+    if (e == hipSuccess && nR) e = hipMemcpy(dR, R, nR * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nS) e = hipMemcpy(dS, S, nS * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "spj_join_host staging");
+    } else {
+        uint64_t cnt = 0;
+        rc = spj_join_device((const spj_tuple_t *)dR, nR, (const spj_tuple_t *)dS, nS, (spj_pair_t *)dO, nS, &cnt,
+                             cfg, nullptr);
+        if (rc == SPJ_OK) {
+            if (cnt > out_cap) {
+                rc = fail(SPJ_EOVERFLOW, "out_cap %llu < %llu pairs", (unsigned long long)out_cap, (unsigned long long)cnt);
+            } else {
+                e = cnt ? hipMemcpy(out, dO, cnt * 8, hipMemcpyDeviceToHost) : hipSuccess;
+                if (e != hipSuccess) rc = hip_fail(e, "D2H");
+                *out_count = cnt;
+            }
+        }
+    }
+    if (dR) (void)hipFree(dR);
+    if (dS) (void)hipFree(dS);
+    if (dO) (void)hipFree(dO);
+    return rc;
+}
+
+int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR, const spj_tuple_t *const *S,
+                          const uint64_t *nS, spj_pair_t *const *out, uint64_t *out_count, const spj_config_t *cfg,
+                          spj_phase_times_t *times)
+{
+    if (n < 0 || (n > 0 && (!R || !nR || !S || !nS || !out || !out_count))) return fail(SPJ_EINVAL, "null argument");
+    std::vector<InstSpec> specs((size_t)n);
+    for (int i = 0; i < n; i++) {
+        int rc = validate_sizes(nR[i], nS[i]);
+        if (rc) return rc;
+        InstSpec &s = specs[(size_t)i];
+        s.R = reinterpret_cast<const uint2 *>(R[i]);
+        s.S = reinterpret_cast<const uint2 *>(S[i]);
+        s.out = reinterpret_cast<uint2 *>(out[i]);
+        s.nR = (uint32_t)nR[i];
+        s.nS = (uint32_t)nS[i];
+        s.cfg = resolve(cfg, nR[i], nS[i], -1.0);
+        rc = check_resolved(s.cfg);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(default_context_mutex());
+    std::vector<uint32_t> counts, errs;
+    PhaseTimes t;
+    hipError_t e = default_context().run(specs, counts, errs, times ? &t : nullptr);
+    if (e != hipSuccess) return hip_fail(e, "batch join");
+    fill_times(t, times);
+    int rc = SPJ_OK;
+    for (int i = 0; i < n; i++) {
+        out_count[i] = counts[(size_t)i];
+        if (rc == SPJ_OK) rc = err_from_flags(errs[(size_t)i], (size_t)i);
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// spj_common.hpp -- descriptors shared by the host engine and the HIP kernels.
+//
+// A "batch" is a set of independent join instances.  A single join is a batch
+// of one; the batched workload (thousands of uneven joins) and the serving
+// loop (many probe batches against resident tables) use the same kernels.
+//
+// Layout in HBM (all 8-byte AoS tuples {uint32 key; uint32 rid}, the
+// reference's kv_pair_t, type_defs.hpp:53-56):
+//   R, S                 caller's inputs (read only)
+//   Rp, Sp               radix-partitioned copies, one region per instance,
+//                        partition p of an instance occupying
+//                        [pstart[p], pstart[p+1]) of its region
+//   counts               per-tile digit counts, scanned in place into the
+//                        tile's global write offsets
+//   out                  (R rid, S rid) pairs, one region per instance with
+//                        capacity |S_i|; exact count in out_count[i]
+#pragma once
+#include <stdint.h>
+
+namespace spj {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kEmptyKey = 0u;          // key 0 = empty / skipped (glb_partition_count.c:192)
+constexpr uint32_t kNoRid = 0xFFFFFFFFu;    // heavy key with no R tuple yet
+constexpr uint32_t kHeavySlots = 4096;      // heavy (key -> R rid) table, LDS resident
+constexpr uint32_t kHeavyMaxKeys = 2048;    // at most half full
+constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
+
+// error bits in err[inst]
+constexpr uint32_t kErrDuplicateKey = 1u;   // a2a_srj_build.c:265-271 (VALIDATION)
+constexpr uint32_t kErrTableOverflow = 2u;  // a sub-table received more tuples than slots
+constexpr uint32_t kErrOutOverflow = 4u;    // more pairs than the output capacity
+
+struct InstDesc {
+    const uint2 *R;          // input R (device)
+    const uint2 *S;          // input S (device)
+    uint2 *Rp;               // partitioned R region
+    uint2 *Sp;               // partitioned S region
+    uint2 *out;              // output pairs region (capacity nS)
+    uint32_t nR, nS;
+    uint32_t bits;           // radix bits (F = 1 << bits)
+    uint32_t heavy;          // 1: heavy table in use for this instance
+    uint64_t pstartR;        // offset (in uint32) of R partition starts (F+1 entries)
+    uint64_t pstartS;        // offset of S partition starts (F+1 entries)
+    uint32_t *heavy_keys;    // kHeavySlots keys (0 = empty)
+    uint32_t *heavy_rids;    // kHeavySlots R rids (kNoRid = none)
+    uint32_t table_log2;     // LDS hash table slots (log2) for this instance
+    uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop
+};
+
+// One tile of one relation of one instance.  Its digit counts live at
+// counts[cnt_off .. cnt_off + F).
+struct TileDesc {
+    uint32_t inst;
+    uint32_t rel;            // 0 = R, 1 = S
+    uint32_t begin, end;     // tuple range inside the relation
+    uint64_t cnt_off;
+};
+
+// One (instance, relation) whose tile counts are scanned together.  Its tiles
+// are t_first .. t_first + t_count - 1, with counts at
+// cnt_off0 + (t - t_first) * F; chunk partials live at part_off; partition
+// starts at pstart_off (F + 1 entries).
+struct ScanSeg {
+    uint32_t inst;
+    uint32_t rel;
+    uint32_t bits;
+    uint32_t t_count;
+    uint64_t cnt_off0;
+    uint64_t part_off;
+    uint64_t pstart_off;
+};
+
+// One block of a column-scan phase: segment, 256-digit chunk, tile chunk.
+struct ScanBlock {
+    uint32_t seg;
+    uint32_t dchunk;
+    uint32_t tchunk;
+    uint32_t pad;
+};
+
+constexpr uint32_t kScanTileChunk = 32;   // tiles per column-scan chunk
+
+// One unit of join work: partition p of instance inst, S range [sb, se)
+// (absolute positions in the instance's partitioned S region).
+struct WorkItem {
+    uint32_t inst;
+    uint32_t part;
+    uint32_t sb, se;
+};
+
+}  // namespace spj

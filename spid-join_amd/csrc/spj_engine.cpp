@@ -1,0 +1,303 @@
+// spj_engine.cpp -- batch planning and phase launch sequence.
+//
+// Phase order follows the reference's plans (execution.hpp:15-157 and the
+// compound operators of queryCompoundFunc.cc): partition R, partition S
+// (with SPID's skew replication folded in as heavy-key handling), then the
+// per-partition join.  Rank threads, barriers and the host shuffle of the
+// reference (pipeline.cc, rotate_and_stream.c) have no counterpart: every
+// phase is a kernel on one stream and partitions are written in place in HBM.
+#include "spj_engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "spj_kernels.hpp"
+
+namespace spj {
+
+#define SPJ_TRY(x)                                  \
+    do {                                            \
+        hipError_t _e = (x);                        \
+        if (_e != hipSuccess) return _e;            \
+    } while (0)
+
+hipError_t DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    want = (want + 4095) & ~size_t(4095);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+const DeviceInfo &device_info()
+{
+    static DeviceInfo info;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+        info.device = dev;
+        info.cus = prop.multiProcessorCount;
+        info.lds_per_block = prop.sharedMemPerBlock;
+        info.arch = prop.gcnArchName;
+    });
+    return info;
+}
+
+static uint32_t ceil_log2(uint64_t x)
+{
+    uint32_t r = 0;
+    while ((uint64_t(1) << r) < x) r++;
+    return r;
+}
+
+Resolved default_params(uint32_t nR, uint32_t nS)
+{
+    Resolved r;
+    r.table_log2 = 14;
+    // Aim at ~8K R tuples per partition: half of a 16K-slot LDS table.
+    uint32_t b = nR > 8192 ? ceil_log2((uint64_t(nR) + 8191) / 8192) : 0;
+    r.bits = std::min<uint32_t>(b, kMaxBits);
+    r.ipt = 16;
+    // Tiles: at least one scatter round; at most 2^24 counters per relation.
+    uint64_t n = std::max(nR, nS);
+    uint64_t t = 8192;
+    while ((n / t) * (uint64_t(1) << r.bits) > (uint64_t(1) << 24)) t <<= 1;
+    r.tile = (uint32_t)t;
+    r.smax = 65536;
+    r.heavy = nS >= (1u << 20) ? 1u : 0u;
+    r.heavy_min = 16;
+    r.alg = 0;
+    return r;
+}
+
+Context::Context()
+{
+    (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+    for (auto &e : ev_) (void)hipEventCreate(&e);
+}
+
+Context::~Context()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_,
+                      &rp_, &sp_, &items_, &ctrl_, &heavy_, &cand_};
+    for (auto *b : bufs) b->release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ctrl layout (uint32): [0] nitems, [1] qhead, [2..3] pad,
+// [4 .. 4+n) out_count, [4+n .. 4+2n) err, [4+2n .. 4+3n) nheavy
+static inline size_t ctrl_words(size_t n) { return 4 + 3 * n; }
+
+hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed)
+{
+    const size_t n = specs.size();
+    h_insts_.assign(n, InstDesc{});
+    h_tiles_.clear();
+    h_segs_.clear();
+    h_ad_.clear();
+    h_b_.clear();
+    max_bits_ = 0;
+    max_table_ = 0;
+    max_ipt_ = 8;
+    any_heavy_ = false;
+
+    // ---- plan: regions, tiles, scan segments
+    uint64_t rp_total = 0, sp_total = 0, cnt_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
+    // one S-range size per batch (the item kernel takes a single value)
+    const uint32_t smax = specs.empty() ? 65536u : specs[0].cfg.smax;
+    uint32_t nheavy_inst = 0;
+    std::vector<uint64_t> rp_off(n), sp_off(n);
+    std::vector<int> heavy_idx(n, -1);
+    for (size_t i = 0; i < n; i++) {
+        const InstSpec &s = specs[i];
+        const Resolved &c = s.cfg;
+        const uint32_t F = 1u << c.bits;
+        max_bits_ = std::max(max_bits_, c.bits);
+        max_table_ = std::max(max_table_, c.table_log2);
+        max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
+        rp_off[i] = rp_total;
+        sp_off[i] = sp_total;
+        rp_total += s.nR;
+        sp_total += s.nS;
+        InstDesc &d = h_insts_[i];
+        d.R = s.R;
+        d.S = s.S;
+        d.out = s.out;
+        d.nR = s.nR;
+        d.nS = s.nS;
+        d.bits = c.bits;
+        d.heavy = (c.heavy && s.nS > 0) ? 1u : 0u;
+        d.table_log2 = c.table_log2;
+        d.alg = c.alg;
+        d.pstartR = pst_total;
+        d.pstartS = pst_total + F + 1;
+        pst_total += 2ull * (F + 1);
+        if (d.heavy) {
+            heavy_idx[i] = (int)nheavy_inst++;
+            any_heavy_ = true;
+        }
+        items_bound += F + (uint64_t)s.nS / smax + 1;
+        for (uint32_t rel = 0; rel < 2; rel++) {
+            const uint32_t len = rel ? s.nS : s.nR;
+            const uint32_t T = (len + c.tile - 1) / c.tile;
+            ScanSeg sg{};
+            sg.inst = (uint32_t)i;
+            sg.rel = rel;
+            sg.bits = c.bits;
+            sg.t_count = T;
+            sg.cnt_off0 = cnt_total;
+            sg.part_off = part_total;
+            sg.pstart_off = rel ? d.pstartS : d.pstartR;
+            const uint32_t seg_id = (uint32_t)h_segs_.size();
+            h_segs_.push_back(sg);
+            for (uint32_t t = 0; t < T; t++) {
+                TileDesc td{};
+                td.inst = (uint32_t)i;
+                td.rel = rel;
+                td.begin = t * c.tile;
+                td.end = std::min<uint64_t>(len, (uint64_t)(t + 1) * c.tile);
+                td.cnt_off = cnt_total + (uint64_t)t * F;
+                h_tiles_.push_back(td);
+            }
+            const uint32_t dch = (F + 255) / 256, tch = (T + kScanTileChunk - 1) / kScanTileChunk;
+            for (uint32_t dc = 0; dc < dch; dc++) {
+                h_b_.push_back(ScanBlock{seg_id, dc, 0, 0});
+                for (uint32_t tc = 0; tc < tch; tc++) h_ad_.push_back(ScanBlock{seg_id, dc, tc, 0});
+            }
+            cnt_total += (uint64_t)T * F;
+            part_total += (uint64_t)tch * F;
+        }
+    }
+    if (h_tiles_.size() > 0x7FFFFFFFull) return hipErrorInvalidValue;
+
+    // ---- buffers
+    SPJ_TRY(insts_.ensure(n * sizeof(InstDesc)));
+    SPJ_TRY(tiles_.ensure(h_tiles_.size() * sizeof(TileDesc)));
+    SPJ_TRY(counts_.ensure(cnt_total * 4));
+    SPJ_TRY(part_.ensure(part_total * 4));
+    SPJ_TRY(pstart_.ensure(pst_total * 4));
+    SPJ_TRY(segs_.ensure(h_segs_.size() * sizeof(ScanSeg)));
+    SPJ_TRY(blk_ad_.ensure(h_ad_.size() * sizeof(ScanBlock)));
+    SPJ_TRY(blk_b_.ensure(h_b_.size() * sizeof(ScanBlock)));
+    SPJ_TRY(rp_.ensure(rp_total * 8));
+    SPJ_TRY(sp_.ensure(sp_total * 8));
+    SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
+    SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
+    if (nheavy_inst) {
+        SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
+        SPJ_TRY(cand_.ensure((size_t)2 * cand_slots() * 4));
+    }
+    for (size_t i = 0; i < n; i++) {
+        InstDesc &d = h_insts_[i];
+        d.Rp = rp_.as<uint2>() + rp_off[i];
+        d.Sp = sp_.as<uint2>() + sp_off[i];
+        if (heavy_idx[i] >= 0) {
+            d.heavy_keys = heavy_.as<uint32_t>() + (size_t)heavy_idx[i] * 2 * kHeavySlots;
+            d.heavy_rids = d.heavy_keys + kHeavySlots;
+        }
+    }
+
+    hipStream_t st = stream_;
+    SPJ_TRY(hipMemcpyAsync(insts_.p, h_insts_.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, st));
+    if (!h_tiles_.empty())
+        SPJ_TRY(hipMemcpyAsync(tiles_.p, h_tiles_.data(), h_tiles_.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    SPJ_TRY(hipMemcpyAsync(segs_.p, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg), hipMemcpyHostToDevice, st));
+    if (!h_ad_.empty())
+        SPJ_TRY(hipMemcpyAsync(blk_ad_.p, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
+    SPJ_TRY(hipMemcpyAsync(blk_b_.p, h_b_.data(), h_b_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
+
+    const InstDesc *d_insts = insts_.as<InstDesc>();
+    uint32_t *ctrl = ctrl_.as<uint32_t>();
+    uint32_t *out_count = ctrl + 4, *err = ctrl + 4 + n, *nheavy = ctrl + 4 + 2 * n;
+    if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
+    SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
+
+    // ---- heavy keys
+    if (nheavy_inst) {
+        SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
+        for (size_t i = 0; i < n; i++) {
+            if (heavy_idx[i] < 0) continue;
+            SPJ_TRY(launch_fill(cand_.as<uint32_t>(), 2ull * cand_slots(), 0u, st));
+            SPJ_TRY(launch_heavy_detect(d_insts, (uint32_t)i, specs[i].nS, 1024, specs[i].cfg.heavy_min,
+                                        cand_.as<uint32_t>(), cand_.as<uint32_t>() + cand_slots(), nheavy + i, st));
+        }
+    }
+    if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
+
+    // ---- partition: histogram, scan, scatter
+    const uint32_t ntiles = (uint32_t)h_tiles_.size();
+    SPJ_TRY(launch_hist(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), max_bits_, st));
+    if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
+    SPJ_TRY(launch_scan(segs_.as<ScanSeg>(), (uint32_t)h_segs_.size(), blk_ad_.as<ScanBlock>(), (uint32_t)h_ad_.size(),
+                        blk_b_.as<ScanBlock>(), (uint32_t)h_b_.size(), counts_.as<uint32_t>(), part_.as<uint32_t>(),
+                        pstart_.as<uint32_t>(), max_bits_, st));
+    if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
+    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_count, err, max_bits_,
+                           max_ipt_, st));
+    if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
+
+    // ---- join
+    SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), smax, items_.as<WorkItem>(), ctrl, st));
+    if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
+    const DeviceInfo &di = device_info();
+    uint32_t grid = (uint32_t)std::max(1, di.cus) * ((max_table_ <= 13) ? 2u : 1u);
+    SPJ_TRY(launch_join(d_insts, pstart_.as<uint32_t>(), items_.as<WorkItem>(), ctrl, ctrl + 1, out_count, err,
+                        max_table_, grid, st));
+    if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
+    return hipSuccess;
+}
+
+hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed)
+{
+    h_ctrl_.resize(ctrl_words(n));
+    SPJ_TRY(hipMemcpyAsync(h_ctrl_.data(), ctrl_.p, ctrl_words(n) * 4, hipMemcpyDeviceToHost, stream_));
+    SPJ_TRY(hipStreamSynchronize(stream_));
+    counts.assign(h_ctrl_.begin() + 4, h_ctrl_.begin() + 4 + n);
+    errs.assign(h_ctrl_.begin() + 4 + n, h_ctrl_.begin() + 4 + 2 * n);
+    if (timed) {
+        float ms[6];
+        for (int i = 0; i < 6; i++) SPJ_TRY(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+        timed->heavy = ms[0];
+        timed->hist = ms[1];
+        timed->scan = ms[2];
+        timed->scatter = ms[3];
+        timed->items = ms[4];
+        timed->join = ms[5];
+        SPJ_TRY(hipEventElapsedTime(&timed->total, ev_[0], ev_[6]));
+    }
+    return hipSuccess;
+}
+
+hipError_t Context::run(const std::vector<InstSpec> &specs, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs,
+                        PhaseTimes *timed)
+{
+    SPJ_TRY(enqueue(specs, timed));
+    return collect(specs.size(), counts, errs, timed);
+}
+
+}  // namespace spj

@@ -1,0 +1,97 @@
+// spj_engine.hpp -- host engine: plans a batch of join instances, owns the
+// device work buffers of one execution context (one HIP stream), launches
+// the phase sequence and reads back exact result counts.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "spj_common.hpp"
+
+namespace spj {
+
+// Per-instance parameters after the tuner / defaults have been applied.
+struct Resolved {
+    uint32_t bits = 0;        // radix bits of the partition pass
+    uint32_t table_log2 = 14; // LDS hash-table slots (log2)
+    uint32_t alg = 0;         // 0 hash (+NL for tiny), 2 sort-merge, 3 nested loop
+    uint32_t heavy = 0;       // heavy-key detection and direct emission
+    uint32_t heavy_min = 16;  // summed sample count for a key to be heavy
+    uint32_t ipt = 16;        // scatter items per thread (round = 512*ipt)
+    uint32_t tile = 65536;    // tuples per tile of the partition pass
+    uint32_t smax = 65536;    // S tuples per work item
+};
+
+struct InstSpec {
+    const uint2 *R = nullptr;
+    uint32_t nR = 0;
+    const uint2 *S = nullptr;
+    uint32_t nS = 0;
+    uint2 *out = nullptr;     // capacity >= nS pairs
+    Resolved cfg;
+};
+
+// Milliseconds per phase (HIP events on the context's stream).
+struct PhaseTimes {
+    float heavy = 0, hist = 0, scan = 0, scatter = 0, items = 0, join = 0, total = 0;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct DeviceInfo {
+    int device = 0;
+    int cus = 0;
+    size_t lds_per_block = 0;
+    std::string arch;
+};
+
+// Query (once) the current device.
+const DeviceInfo &device_info();
+
+class Context {
+public:
+    Context();
+    ~Context();
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+
+    hipStream_t stream() const { return stream_; }
+
+    // Run a batch.  counts/errs receive per-instance results.  When timed is
+    // non-null, events bracket every phase.  Returns hipSuccess or the first
+    // HIP error; join-level errors are in errs.
+    hipError_t run(const std::vector<InstSpec> &specs, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs,
+                   PhaseTimes *timed);
+
+    // Enqueue only (no host synchronisation): results land in the device
+    // arrays returned by out_counts()/errs() when the stream reaches them.
+    hipError_t enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed);
+    hipError_t collect(size_t ninst, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed);
+
+private:
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[8] = {};
+    DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_;
+    std::vector<InstDesc> h_insts_;
+    std::vector<TileDesc> h_tiles_;
+    std::vector<ScanSeg> h_segs_;
+    std::vector<ScanBlock> h_ad_, h_b_;
+    std::vector<uint32_t> h_ctrl_;
+    uint32_t max_bits_ = 0, max_table_ = 0;
+    int max_ipt_ = 16;
+    bool any_heavy_ = false;
+};
+
+// Default parameters for an instance of the given sizes (no tuner entry).
+Resolved default_params(uint32_t nR, uint32_t nS);
+
+}  // namespace spj

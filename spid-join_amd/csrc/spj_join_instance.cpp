@@ -1,0 +1,103 @@
+// spj_join_instance.cpp -- pidjoin::JoinInstance on top of the C ABI.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "pidjoin.hpp"
+
+namespace pidjoin {
+
+namespace {
+
+spj_join_design_t to_c(const join_design_t &d)
+{
+    spj_join_design_t c;
+    c.input_size = d.input_size;
+    c.rank_num = d.rank_num;
+    c.zipf_factor = d.zipf_factor;
+    c.rs_ratio = d.rs_ratio;
+    c.rank_set = d.rank_set;
+    c.bank_set = d.bank_set;
+    c.join_alg = d.join_alg;
+    return c;
+}
+
+void check(int rc, const char *what)
+{
+    if (rc != SPJ_OK) throw std::runtime_error(std::string(what) + ": " + spj_last_error());
+}
+
+}  // namespace
+
+JoinInstance::JoinInstance() : JoinInstance(16, 1, 16) {}
+
+JoinInstance::JoinInstance(int n) : JoinInstance(n, 1, n) {}
+
+JoinInstance::JoinInstance(int n, int groups, int per_group)
+    : num_rank_allocated(n), num_rankgroup(groups), num_rank_in_rankgroup(per_group), h_(nullptr)
+{
+    h_ = spj_create(n, groups, per_group);
+    if (!h_) throw std::runtime_error(std::string("JoinInstance: ") + spj_last_error());
+}
+
+JoinInstance::~JoinInstance() { spj_destroy(h_); }
+
+void JoinInstance::SetRankAllocated(int n) { num_rank_allocated = n; }
+
+void JoinInstance::LoadColumn(void *data_ptr, int64_t num_tuples, std::string col_name)
+{
+    check(spj_load_column(h_, data_ptr, num_tuples, col_name.c_str()), "LoadColumn");
+}
+
+void JoinInstance::PrepareJoin(join_design_t d)
+{
+    spj_join_design_t c = to_c(d);
+    check(spj_prepare(h_, &c), "PrepareJoin");
+}
+
+void JoinInstance::StartJoin(join_design_t d)
+{
+    spj_join_design_t c = to_c(d);
+    check(spj_start(h_, &c), "StartJoin");
+}
+
+ResultBuffers_t JoinInstance::FinishJoin()
+{
+    spj_pair_t *pairs = nullptr;
+    uint64_t n = 0;
+    check(spj_finish(h_, &pairs, &n), "FinishJoin");
+    result_bufferpool.first.assign(1, reinterpret_cast<kv_pair_t *>(pairs));
+    result_bufferpool.second.assign(1, (int64_t)(n * sizeof(kv_pair_t)));
+    return result_bufferpool;
+}
+
+void JoinInstance::ExecuteJoin(std::string join_type, join_design_t d)
+{
+    if (join_type != "phj" && join_type != "smj") throw std::runtime_error("Not supported join algorithm " + join_type);
+    d.join_alg = (join_type == "smj") ? JOIN_DESIGN_JOIN_ALGORITHM_SMJ : JOIN_DESIGN_JOIN_ALGORITHM_PHJ;
+    StartJoin(d);
+    FinishJoin();
+}
+
+// Timeline of the last join, one line per phase:
+// "rank_group-0-idx-start_ms-stop_ms-PHASE" (queryHostFunc.cc:183-303 prints
+// rg-rank-idx-start-stop-op-inputs per plan node).
+void JoinInstance::RecordLogs(struct timespec start, int rank_group)
+{
+    (void)start;
+    spj_phase_times_t t;
+    if (spj_last_phase_times(h_, &t) != SPJ_OK) return;
+    const char *names[] = {"HEAVY_KEY_DETECT", "GLB_PARTITION_COUNT", "CALCULATE_PAGE_HISTOGRAM",
+                           "GLB_PARTITION_PACKET", "MAKE_WORK_ITEMS", "JOIN"};
+    float v[] = {t.heavy_ms, t.hist_ms, t.scan_ms, t.scatter_ms, t.items_ms, t.join_ms};
+    float at = 0;
+    for (int i = 0; i < 6; i++) {
+        printf("%d-0-%d-%.4f-%.4f-%s\n", rank_group, i, at, at + v[i], names[i]);
+        at += v[i];
+    }
+    fflush(stdout);
+}
+
+}  // namespace pidjoin

@@ -1,0 +1,874 @@
+// spj_kernels.hip -- CDNA4 (gfx950) kernels for the skew-resistant equi-join.
+//
+// Phase map (reference operator -> kernel):
+//   GLB_PARTITION_COUNT (glb_partition_count.c)          -> k_hist
+//   CALCULATE_PAGE_HISTOGRAM (queryHostFunc.cc:875-960)   -> k_scan_a/b/c/d
+//   GLB_PARTITION_PACKET + ROTATE_AND_STREAM              -> k_scatter
+//   BG_BROADCAST / CHIPWISE_BROADCAST (SPID replication)  -> k_heavy_sample,
+//        k_heavy_select, heavy capture in k_hist (R) and
+//        direct heavy emission in k_scatter (S)
+//   PHJ_BUILD + PHJ_PROBE (a2a_srj_build.c,
+//        a2a_srj_probe_inner_mt.c)                         -> k_join (hash)
+//   MPSM_JOIN_* (mpsm_join_*.c)                           -> k_join (sort-merge)
+//   NESTED_LOOP_JOIN (nested_loop_join.c)                 -> k_join (nested loop)
+//
+// All work is integer: hashing, LDS atomics, prefix sums, coalesced copies.
+// The bound is HBM bandwidth; no MFMA.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spj_common.hpp"
+#include "spj_kernels.hpp"
+
+namespace spj {
+
+// ---------------------------------------------------------------------------
+// Hashes (device).  Same functions as the reference's src/dpu/src/hash.c.
+// ---------------------------------------------------------------------------
+
+// Paul Hsieh's SuperFastHash of the 4 little-endian key bytes, initial hash
+// = length 4 (hash.c:117-166).  Its low bits select the global partition.
+__device__ __forceinline__ uint32_t sfh(uint32_t key)
+{
+    uint32_t h = 4u + (key & 0xFFFFu);
+    uint32_t tmp = ((key >> 16) << 11) ^ h;
+    h = (h << 16) ^ tmp;
+    h += h >> 11;
+    h ^= h << 3;
+    h += h >> 5;
+    h ^= h << 4;
+    h += h >> 17;
+    h ^= h << 25;
+    h += h >> 6;
+    return h;
+}
+
+// CRC-32 (reflected, polynomial 0xEDB88320) table built into LDS by the
+// workgroup itself: 256 threads, 8 shift/xor steps each.
+__device__ __forceinline__ void build_crc_lds(uint32_t *crc)
+{
+    for (uint32_t n = threadIdx.x; n < 256; n += blockDim.x) {
+        uint32_t c = n;
+#pragma unroll
+        for (int k = 0; k < 8; k++) c = (c & 1u) ? (0xEDB88320u ^ (c >> 1)) : (c >> 1);
+        crc[n] = c;
+    }
+}
+
+// join_hash: (0xd9d65adc & T[b0]) ^ T[b1] ^ T[b2] ^ T[b3] (hash.c:72-83).
+// Its low bits pick the LDS hash-table slot, as in a2a_srj_build.c:209-212.
+__device__ __forceinline__ uint32_t join_hash(const uint32_t *crc, uint32_t k)
+{
+    return (0xd9d65adcu & crc[k & 0xFFu]) ^ crc[(k >> 8) & 0xFFu] ^ crc[(k >> 16) & 0xFFu] ^ crc[k >> 24];
+}
+
+// Heavy-table slot: multiplicative hash (the table is this build's own).
+__device__ __forceinline__ uint32_t heavy_slot(uint32_t key)
+{
+    return (key * 0x9E3779B1u) >> (32 - 12);   // kHeavySlots = 4096
+}
+
+// Sub-table of a partition that needs Q > 1 LDS tables: high SFH bits, which
+// do not depend on the radix bits that chose the partition.
+__device__ __forceinline__ uint32_t sub_of(uint32_t h, uint32_t bits, uint32_t Q)
+{
+    return __umulhi((h >> bits) * 0x9E3779B1u, Q);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// In-place exclusive scan of a[0..n) in LDS by the whole workgroup; returns
+// the total.  wtmp needs blockDim/64 + 1 words.
+__device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wtmp)
+{
+    const uint32_t nt = blockDim.x, tid = threadIdx.x;
+    const uint32_t nw = nt / kWave, wid = tid / kWave, lane = lane_id();
+    uint32_t per = (n + nt - 1) / nt;
+    uint32_t b = tid * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += a[i];
+    uint32_t x = s;
+#pragma unroll
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wtmp[wid] = x;
+    __syncthreads();
+    if (tid < kWave) {
+        uint32_t v = (tid < nw) ? wtmp[tid] : 0u;
+        uint32_t z = v;
+#pragma unroll
+        for (uint32_t off = 1; off < kWave; off <<= 1) {
+            uint32_t y = __shfl_up(z, off, kWave);
+            if (tid >= off) z += y;
+        }
+        if (tid < nw) wtmp[tid] = z - v;
+        if (tid == kWave - 1) wtmp[nw] = z;
+    }
+    __syncthreads();
+    uint32_t ex = wtmp[wid] + x - s;
+    for (uint32_t i = b; i < e; i++) { uint32_t v = a[i]; a[i] = ex; ex += v; }
+    uint32_t total = wtmp[nw];
+    __syncthreads();
+    return total;
+}
+
+// Tile index of block b when T tiles are spread over 8 XCDs: the hardware
+// hands block b to XCD b % 8, so XCD x gets one contiguous run of tiles and
+// neighbouring partition runs are written through the same L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
+{
+    uint32_t x = b & 7u, j = b >> 3, q = T >> 3, r = T & 7u;
+    return x * q + min(x, r) + j;
+}
+
+// Load the instance's heavy table into LDS.
+__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr)
+{
+    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
+        hk[i] = I.heavy_keys[i];
+        hr[i] = I.heavy_rids[i];
+    }
+}
+
+// Slot of key in the LDS heavy table, or -1.
+__device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key)
+{
+    uint32_t s = heavy_slot(key);
+    for (uint32_t step = 0; step < kHeavySlots; step++) {
+        uint32_t k = hk[s];
+        if (k == key) return (int)s;
+        if (k == kEmptyKey) return -1;
+        s = (s + 1u) & (kHeavySlots - 1u);
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// k_fill: small fill kernel (graph-capture safe alternative to memset).
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// k_hist: digit histogram of one tile (glb_partition_count.c:119-371).
+// For R with a heavy table it also records each heavy key's R rid.  For S
+// with a heavy table, heavy tuples are not counted (they never go to a
+// partition).  Key 0 is skipped everywhere.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
+                                              uint32_t ntiles, uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t t = xcd_tile(blockIdx.x, ntiles);
+    const TileDesc td = tiles[t];
+    const InstDesc &I = insts[td.inst];
+    const uint32_t F = 1u << I.bits, mask = F - 1u;
+    uint32_t *hist = lds;                 // F
+    uint32_t *hk = lds + F;               // kHeavySlots
+    uint32_t *hr = hk + kHeavySlots;      // kHeavySlots
+    const bool heavy = I.heavy != 0;
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
+    if (heavy) load_heavy(I, hk, hr);
+    __syncthreads();
+    const uint2 *in = td.rel ? I.S : I.R;
+    if (!heavy) {
+        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
+            uint2 v = in[i];
+            if (v.x != kEmptyKey) atomicAdd(&hist[sfh(v.x) & mask], 1u);
+        }
+    } else if (td.rel == 0) {
+        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
+            uint2 v = in[i];
+            if (v.x == kEmptyKey) continue;
+            atomicAdd(&hist[sfh(v.x) & mask], 1u);
+            int s = heavy_find(hk, v.x);
+            if (s >= 0) I.heavy_rids[s] = v.y;   // R keys are unique: one writer per slot
+        }
+    } else {
+        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
+            uint2 v = in[i];
+            if (v.x == kEmptyKey || heavy_find(hk, v.x) >= 0) continue;
+            atomicAdd(&hist[sfh(v.x) & mask], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t *dst = counts + td.cnt_off;
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) dst[i] = hist[i];
+}
+
+// ---------------------------------------------------------------------------
+// Column scan of tile counts into global write offsets
+// (the page histogram + cross-rank prefix of queryHostFunc.cc:329-368,
+// 1574-1769, done on the device and without max-padding).
+//   a: per tile chunk, exclusive prefix down each digit column; chunk sums
+//   b: per digit, exclusive prefix over chunk sums; digit totals
+//   c: per segment, exclusive prefix over digits -> partition starts
+//   d: add chunk base and partition start to every tile offset
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scan_a(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
+                                                uint32_t *__restrict__ counts, uint32_t *__restrict__ part)
+{
+    const ScanBlock bk = blocks[blockIdx.x];
+    const ScanSeg sg = segs[bk.seg];
+    const uint32_t F = 1u << sg.bits;
+    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
+    if (d >= F) return;
+    const uint32_t t0 = bk.tchunk * kScanTileChunk, t1 = min(sg.t_count, t0 + kScanTileChunk);
+    uint32_t run = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        uint32_t *c = counts + sg.cnt_off0 + (uint64_t)t * F + d;
+        uint32_t v = *c;
+        *c = run;
+        run += v;
+    }
+    part[sg.part_off + (uint64_t)bk.tchunk * F + d] = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan_b(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
+                                                uint32_t *__restrict__ part, uint32_t *__restrict__ pstart)
+{
+    const ScanBlock bk = blocks[blockIdx.x];
+    const ScanSeg sg = segs[bk.seg];
+    const uint32_t F = 1u << sg.bits;
+    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
+    if (d >= F) return;
+    const uint32_t nch = (sg.t_count + kScanTileChunk - 1) / kScanTileChunk;
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < nch; c++) {
+        uint32_t *p = part + sg.part_off + (uint64_t)c * F + d;
+        uint32_t v = *p;
+        *p = run;
+        run += v;
+    }
+    pstart[sg.pstart_off + d] = run;   // digit total; turned into starts by k_scan_c
+}
+
+__global__ __launch_bounds__(1024) void k_scan_c(const ScanSeg *__restrict__ segs, uint32_t *__restrict__ pstart)
+{
+    __shared__ uint32_t wtmp[1024 / kWave + 1];
+    extern __shared__ uint32_t lds[];
+    const ScanSeg sg = segs[blockIdx.x];
+    const uint32_t F = 1u << sg.bits;
+    uint32_t *p = pstart + sg.pstart_off;
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) lds[i] = p[i];
+    __syncthreads();
+    uint32_t total = block_exclusive_scan(lds, F, wtmp);
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) p[i] = lds[i];
+    if (threadIdx.x == 0) p[F] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_d(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
+                                                uint32_t *__restrict__ counts, const uint32_t *__restrict__ part,
+                                                const uint32_t *__restrict__ pstart)
+{
+    const ScanBlock bk = blocks[blockIdx.x];
+    const ScanSeg sg = segs[bk.seg];
+    const uint32_t F = 1u << sg.bits;
+    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
+    if (d >= F) return;
+    const uint32_t base = part[sg.part_off + (uint64_t)bk.tchunk * F + d] + pstart[sg.pstart_off + d];
+    const uint32_t t0 = bk.tchunk * kScanTileChunk, t1 = min(sg.t_count, t0 + kScanTileChunk);
+    for (uint32_t t = t0; t < t1; t++) counts[sg.cnt_off0 + (uint64_t)t * F + d] += base;
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: LDS-staged radix scatter (glb_partition_packet.c:355-589 writes
+// one 8-byte tuple per mram_write; here a round of NT*IPT tuples is counting-
+// sorted by digit in LDS and each digit run is written with consecutive
+// lanes on consecutive addresses).  For S with a heavy table, heavy tuples
+// form bucket F and are emitted straight to the output as (R rid, S rid).
+// ---------------------------------------------------------------------------
+template <int IPT>
+__global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
+                                                 uint32_t ntiles, const uint32_t *__restrict__ counts,
+                                                 uint32_t *__restrict__ out_count, uint32_t *__restrict__ err)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t NT = 512, ROUND = NT * IPT;
+    constexpr uint32_t DROP = 0xFFFFFFFFu;
+    const uint32_t t = xcd_tile(blockIdx.x, ntiles);
+    const TileDesc td = tiles[t];
+    const InstDesc &I = insts[td.inst];
+    const uint32_t F = 1u << I.bits, mask = F - 1u;
+    const bool heavy = (I.heavy != 0) && (td.rel == 1);
+    // LDS layout
+    uint2 *stage = reinterpret_cast<uint2 *>(lds);                  // ROUND tuples
+    uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
+    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // F + 2: counts -> starts
+    uint32_t *off = lst + F + 2;                                    // F running offsets
+    uint32_t *wtmp = off + F;                                       // NT/64 + 1
+    uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
+    uint32_t *hk = misc + 4;                                        // kHeavySlots
+    uint32_t *hr = hk + kHeavySlots;
+
+    const uint32_t *cnt = counts + td.cnt_off;
+    for (uint32_t i = threadIdx.x; i < F; i += NT) off[i] = cnt[i];
+    if (heavy) load_heavy(I, hk, hr);
+    const uint2 *in = td.rel ? I.S : I.R;
+    uint2 *outp = td.rel ? I.Sp : I.Rp;
+
+    for (uint32_t rb = td.begin; rb < td.end; rb += ROUND) {
+        const uint32_t rn = min(ROUND, td.end - rb);
+        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
+        __syncthreads();
+        uint2 v[IPT];
+        uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            const uint32_t li = j * NT + threadIdx.x;
+            dg[j] = DROP;
+            if (li < rn) {
+                v[j] = in[rb + li];
+                if (v[j].x != kEmptyKey) {
+                    if (heavy) {
+                        int s = heavy_find(hk, v[j].x);
+                        if (s >= 0) {
+                            // heavy: bucket F, unless the key has no R tuple
+                            if (hr[s] != kNoRid) { dg[j] = F; v[j].x = hr[s]; }
+                        } else {
+                            dg[j] = sfh(v[j].x) & mask;
+                        }
+                    } else {
+                        dg[j] = sfh(v[j].x) & mask;
+                    }
+                }
+            }
+            if (dg[j] != DROP) rk[j] = atomicAdd(&lst[dg[j]], 1u);
+        }
+        __syncthreads();
+        const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            if (dg[j] != DROP) {
+                const uint32_t p = lst[dg[j]] + rk[j];
+                stage[p] = v[j];
+                sdig[p] = (uint16_t)dg[j];
+            }
+        }
+        const uint32_t hstart = heavy ? lst[F] : total;
+        if (heavy && threadIdx.x == 0) {
+            const uint32_t hn = total - hstart;
+            misc[0] = hn ? atomicAdd(&out_count[td.inst], hn) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < hstart; i += NT) {
+            const uint32_t d = sdig[i];
+            outp[off[d] + (i - lst[d])] = stage[i];
+        }
+        if (heavy) {
+            const uint32_t hb = misc[0];
+            for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
+                const uint32_t o = hb + (i - hstart);
+                const uint2 sv = stage[i];   // .x already holds the R rid
+                if (o < I.nS) I.out[o] = make_uint2(sv.x, sv.y);
+                else atomicOr(&err[td.inst], kErrOutOverflow);
+            }
+        }
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < F; d += NT) {
+            const uint32_t nx = (d + 1 < F) ? lst[d + 1] : hstart;
+            off[d] += nx - lst[d];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Heavy-key detection (the skew step SPID handles with bank-set / rank-set
+// replication, glb_chipwise_partition_count.c:251-255 and
+// queryHostFunc.cc:1181-1419).  Sample blocks of S are counted exactly in an
+// LDS table; keys seen at least kLocalMin times in a block are merged into a
+// global candidate map, and k_heavy_select keeps candidates whose summed
+// sample count reaches min_count.  The hot keys' R tuples are then held by
+// every workgroup (the heavy table in LDS) and their S tuples are joined in
+// whatever workgroup reads them: R replicated to all workgroups, S split by
+// position -- the same multiset as the plain join.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSampleBlock = 8192;
+constexpr uint32_t kSampleSlots = 16384;
+constexpr uint32_t kLocalMin = 4;
+constexpr uint32_t kCandSlots = 16384;
+
+__global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict__ insts, uint32_t inst,
+                                                      uint32_t stride_blocks, uint32_t *__restrict__ cand_keys,
+                                                      uint32_t *__restrict__ cand_cnt)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t *sk = lds;
+    uint32_t *sc = lds + kSampleSlots;
+    const InstDesc &I = insts[inst];
+    for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
+    __syncthreads();
+    const uint64_t b = (uint64_t)blockIdx.x * stride_blocks * kSampleBlock;
+    const uint32_t e = (uint32_t)min<uint64_t>(I.nS, b + kSampleBlock);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t key = I.S[i].x;
+        if (key == kEmptyKey) continue;
+        uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
+        for (uint32_t step = 0; step < kSampleSlots; step++) {
+            uint32_t old = atomicCAS(&sk[s], kEmptyKey, key);
+            if (old == kEmptyKey || old == key) { atomicAdd(&sc[s], 1u); break; }
+            s = (s + 1u) & (kSampleSlots - 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
+        if (sc[i] < kLocalMin) continue;
+        const uint32_t key = sk[i];
+        uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
+        for (uint32_t step = 0; step < kCandSlots; step++) {
+            uint32_t old = atomicCAS(&cand_keys[s], kEmptyKey, key);
+            if (old == kEmptyKey || old == key) { atomicAdd(&cand_cnt[s], sc[i]); break; }
+            s = (s + 1u) & (kCandSlots - 1u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restrict__ insts, uint32_t inst,
+                                                       const uint32_t *__restrict__ cand_keys,
+                                                       const uint32_t *__restrict__ cand_cnt, uint32_t min_count,
+                                                       uint32_t *__restrict__ nheavy)
+{
+    __shared__ uint32_t n;
+    const InstDesc &I = insts[inst];
+    if (threadIdx.x == 0) n = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kCandSlots; i += blockDim.x) {
+        if (cand_keys[i] == kEmptyKey || cand_cnt[i] < min_count) continue;
+        if (atomicAdd(&n, 1u) >= kHeavyMaxKeys) continue;
+        const uint32_t key = cand_keys[i];
+        uint32_t s = heavy_slot(key);
+        for (uint32_t step = 0; step < kHeavySlots; step++) {
+            uint32_t old = atomicCAS(&I.heavy_keys[s], kEmptyKey, key);
+            if (old == kEmptyKey) { I.heavy_rids[s] = kNoRid; break; }
+            if (old == key) break;
+            s = (s + 1u) & (kHeavySlots - 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *nheavy = min(n, kHeavyMaxKeys);
+}
+
+// ---------------------------------------------------------------------------
+// k_make_items: the device work queue.  For every partition with tuples on
+// both sides, ceil(|S_p| / smax) items split its S range; all instances of a
+// batch append to one dense list.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
+                                                     uint32_t smax, WorkItem *__restrict__ items,
+                                                     uint32_t *__restrict__ nitems)
+{
+    __shared__ uint32_t wtmp[1024 / kWave + 1];
+    __shared__ uint32_t cnt[1024];
+    __shared__ uint32_t base;
+    const uint32_t inst = blockIdx.x;
+    const InstDesc &I = insts[inst];
+    const uint32_t F = 1u << I.bits;
+    const uint32_t *pr = pstart + I.pstartR, *ps = pstart + I.pstartS;
+    for (uint32_t p0 = 0; p0 < F; p0 += 1024) {
+        const uint32_t p = p0 + threadIdx.x;
+        uint32_t nr = 0, ns = 0, c = 0;
+        if (p < F) {
+            nr = pr[p + 1] - pr[p];
+            ns = ps[p + 1] - ps[p];
+            c = (nr && ns) ? (ns + smax - 1) / smax : 0u;
+        }
+        cnt[threadIdx.x] = c;
+        __syncthreads();
+        const uint32_t tot = block_exclusive_scan(cnt, 1024, wtmp);
+        if (threadIdx.x == 0) base = tot ? atomicAdd(nitems, tot) : 0u;
+        __syncthreads();
+        uint32_t o = base + cnt[threadIdx.x];
+        for (uint32_t k = 0; k < c; k++) {
+            WorkItem w;
+            w.inst = inst;
+            w.part = p;
+            w.sb = ps[p] + k * smax;
+            w.se = min(ps[p + 1], w.sb + smax);
+            items[o + k] = w;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_join: persistent workgroups pop items from the queue.  Per item the
+// algorithm is chosen by partition size and the instance's setting:
+//   hash (a2a_srj_build.c + a2a_srj_probe_inner_mt.c): R_p in an LDS open-
+//     addressing table, slot = join_hash & (slots-1), linear probing, key 0 =
+//     empty; a partition larger than one table is split into Q sub-tables by
+//     high hash bits, each probed by the S tuples of its sub-range;
+//   nested loop (nested_loop_join.c) for tiny R_p;
+//   sort-merge (mpsm_join_sort_probe.c): R_p and S chunks bitonic-sorted in
+//     LDS, then a per-thread merge.
+// Matches are counted per wave with ballots and appended with one global
+// atomic per round, so the output has exact counts and no padding.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kJoinNT = 512;
+constexpr uint32_t kJoinIPT = 8;
+constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
+constexpr uint32_t kSortChunk = 8192;       // sort-merge chunk (u64 words) per side
+
+struct JoinShared {
+    uint32_t item;
+    uint32_t round_total;
+    uint32_t round_base;
+    uint32_t pad;
+};
+
+// Emit the matches held by this thread for one round: rr[j]/sr[j] valid when
+// (hit >> j) & 1.  All threads of the block must call it.
+template <int IPT>
+__device__ __forceinline__ void emit_round(const uint32_t *rr, const uint32_t *sr, uint32_t hit, JoinShared *sh,
+                                           uint32_t *out_count_i, uint2 *out, uint32_t cap, uint32_t *err_i)
+{
+    uint64_t m[IPT];
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        m[j] = __ballot((hit >> j) & 1u);
+        wtot += __popcll(m[j]);
+    }
+    uint32_t wbase = 0;
+    if (lane_id() == 0 && wtot) wbase = atomicAdd(&sh->round_total, wtot);
+    wbase = __shfl(wbase, 0, kWave);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = sh->round_total;
+        sh->round_base = tot ? atomicAdd(out_count_i, tot) : 0u;
+        sh->round_total = 0;
+    }
+    __syncthreads();
+    uint32_t o = sh->round_base + wbase;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        if ((hit >> j) & 1u) {
+            const uint32_t pos = o + mbcnt(m[j]);
+            if (pos < cap) out[pos] = make_uint2(rr[j], sr[j]);
+            else atomicOr(err_i, kErrOutOverflow);
+        }
+        o += __popcll(m[j]);
+    }
+}
+
+__device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* power of two */)
+{
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t x = a[i], y = a[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { a[i] = y; a[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
+                                                  const WorkItem *__restrict__ items, const uint32_t *__restrict__ nitems,
+                                                  uint32_t *__restrict__ qhead, uint32_t *__restrict__ out_count,
+                                                  uint32_t *__restrict__ err, uint32_t max_table_log2)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ JoinShared sh;
+    __shared__ uint32_t crc[256];
+    __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
+    constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
+    build_crc_lds(crc);
+    if (threadIdx.x == 0) sh.round_total = 0;
+    const uint32_t total_items = *nitems;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh.item = atomicAdd(qhead, 1u);
+        __syncthreads();
+        const uint32_t it = sh.item;
+        if (it >= total_items) break;
+        const WorkItem w = items[it];
+        const InstDesc &I = insts[w.inst];
+        const uint32_t rb = pstart[I.pstartR + w.part], re = pstart[I.pstartR + w.part + 1];
+        const uint32_t nr = re - rb;
+        uint32_t alg = I.alg;
+        if (alg == 0 && nr <= kNLMax) alg = 3;
+        uint32_t *ocnt = out_count + w.inst;
+        uint32_t *erri = err + w.inst;
+
+        if (alg == 3) {
+            // ---------------- nested loop: R chunks in LDS, every S tuple
+            // compares against every R key of the chunk (first match wins)
+            uint32_t *rk = lds, *rv = lds + 4096;
+            for (uint32_t c0 = 0; c0 < nr; c0 += 4096) {
+                const uint32_t cn = min(4096u, nr - c0);
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = I.Rp[rb + c0 + i]; rk[i] = r.x; rv[i] = r.y; }
+                __syncthreads();
+                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                    uint32_t rr[IPT], sr[IPT], hit = 0;
+#pragma unroll
+                    for (int j = 0; j < (int)IPT; j++) {
+                        const uint32_t si = s0 + j * NT + threadIdx.x;
+                        if (si < w.se) {
+                            const uint2 sv = I.Sp[si];
+                            for (uint32_t i = 0; i < cn; i++) {
+                                if (rk[i] == sv.x) { rr[j] = rv[i]; sr[j] = sv.y; hit |= 1u << j; break; }
+                            }
+                        }
+                    }
+                    emit_round<IPT>(rr, sr, hit, &sh, ocnt, I.out, I.nS, erri);
+                }
+            }
+            continue;
+        }
+
+        if (alg == 2) {
+            // ---------------- sort-merge: R chunk sorted once, each S chunk
+            // sorted and merged against it (S tuple takes the equal R key's rid)
+            uint64_t *ra = reinterpret_cast<uint64_t *>(lds);
+            uint64_t *sa = ra + kSortChunk;
+            uint32_t *cnts = reinterpret_cast<uint32_t *>(sa + kSortChunk);   // NT
+            for (uint32_t c0 = 0; c0 < nr; c0 += kSortChunk) {
+                const uint32_t cn = min(kSortChunk, nr - c0);
+                uint32_t np = 1;
+                while (np < cn) np <<= 1;
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < np; i += NT) {
+                    if (i < cn) { uint2 r = I.Rp[rb + c0 + i]; ra[i] = ((uint64_t)r.x << 32) | r.y; }
+                    else ra[i] = ~0ull;
+                }
+                __syncthreads();
+                bitonic_sort_lds(ra, np);
+                for (uint32_t s0 = w.sb; s0 < w.se; s0 += kSortChunk) {
+                    const uint32_t sn = min(kSortChunk, w.se - s0);
+                    uint32_t sp = 1;
+                    while (sp < sn) sp <<= 1;
+                    for (uint32_t i = threadIdx.x; i < sp; i += NT) {
+                        if (i < sn) { uint2 s = I.Sp[s0 + i]; sa[i] = ((uint64_t)s.x << 32) | s.y; }
+                        else sa[i] = ~0ull;
+                    }
+                    __syncthreads();
+                    bitonic_sort_lds(sa, sp);
+                    // merge: thread owns S[a, b); start from lower_bound in R
+                    const uint32_t per = (sn + NT - 1) / NT;
+                    const uint32_t a = min(sn, threadIdx.x * per), b = min(sn, a + per);
+                    uint32_t c = 0;
+                    uint32_t r0 = 0;
+                    if (a < b) {
+                        const uint32_t k0 = (uint32_t)(sa[a] >> 32);
+                        uint32_t lo = 0, hi = cn;
+                        while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if ((uint32_t)(ra[mid] >> 32) < k0) lo = mid + 1; else hi = mid; }
+                        r0 = lo;
+                        uint32_t r = r0;
+                        for (uint32_t i = a; i < b; i++) {
+                            const uint32_t k = (uint32_t)(sa[i] >> 32);
+                            while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
+                            if (r < cn && (uint32_t)(ra[r] >> 32) == k) c++;
+                        }
+                    }
+                    cnts[threadIdx.x] = c;
+                    __syncthreads();
+                    const uint32_t tot = block_exclusive_scan(cnts, NT, wtmp);
+                    if (threadIdx.x == 0) sh.round_base = tot ? atomicAdd(ocnt, tot) : 0u;
+                    __syncthreads();
+                    uint32_t o = sh.round_base + cnts[threadIdx.x];
+                    if (a < b) {
+                        uint32_t r = r0;
+                        for (uint32_t i = a; i < b; i++) {
+                            const uint32_t k = (uint32_t)(sa[i] >> 32);
+                            while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
+                            if (r < cn && (uint32_t)(ra[r] >> 32) == k) {
+                                if (o < I.nS) I.out[o] = make_uint2((uint32_t)ra[r], (uint32_t)sa[i]);
+                                else atomicOr(erri, kErrOutOverflow);
+                                o++;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            continue;
+        }
+
+        // ---------------- hash join
+        const uint32_t maxslots = 1u << min(I.table_log2, max_table_log2);
+        const uint32_t cap = (maxslots / 4u) * 3u;
+        const uint32_t Q = (nr + cap - 1) / cap;
+        const uint32_t nsub = (nr + Q - 1) / Q;
+        uint32_t slots = 64;
+        while (slots < 2 * nsub && slots < maxslots) slots <<= 1;
+        const uint32_t smask = slots - 1u;
+        uint32_t *tk = lds, *tv = lds + slots;
+        for (uint32_t q = 0; q < Q; q++) {
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < slots; i += NT) tk[i] = kEmptyKey;
+            __syncthreads();
+            // build
+            for (uint32_t i = rb + threadIdx.x; i < re; i += NT) {
+                const uint2 r = I.Rp[i];
+                if (Q > 1 && sub_of(sfh(r.x), I.bits, Q) != q) continue;
+                uint32_t h = join_hash(crc, r.x) & smask;
+                uint32_t step = 0;
+                for (; step < slots; step++) {
+                    const uint32_t old = atomicCAS(&tk[h], kEmptyKey, r.x);
+                    if (old == kEmptyKey) { tv[h] = r.y; break; }
+                    if (old == r.x) { atomicOr(erri, kErrDuplicateKey); break; }
+                    h = (h + 1u) & smask;
+                }
+                if (step == slots) atomicOr(erri, kErrTableOverflow);
+            }
+            __syncthreads();
+            // probe
+            for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                uint32_t rr[IPT], sr[IPT], hit = 0;
+#pragma unroll
+                for (int j = 0; j < (int)IPT; j++) {
+                    const uint32_t si = s0 + j * NT + threadIdx.x;
+                    if (si < w.se) {
+                        const uint2 sv = I.Sp[si];
+                        if (Q == 1 || sub_of(sfh(sv.x), I.bits, Q) == q) {
+                            uint32_t h = join_hash(crc, sv.x) & smask;
+                            for (uint32_t step = 0; step < slots; step++) {
+                                const uint32_t k = tk[h];
+                                if (k == sv.x) { rr[j] = tv[h]; sr[j] = sv.y; hit |= 1u << j; break; }
+                                if (k == kEmptyKey) break;
+                                h = (h + 1u) & smask;
+                            }
+                        }
+                    }
+                }
+                emit_round<IPT>(rr, sr, hit, &sh, ocnt, I.out, I.nS, erri);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static hipError_t set_lds(const void *fn, size_t bytes)
+{
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+size_t lds_hist(uint32_t max_bits) { return ((size_t(1) << max_bits) + 2 * kHeavySlots) * 4; }
+
+size_t lds_scatter(uint32_t max_bits, int ipt)
+{
+    const size_t F = size_t(1) << max_bits, R = 512u * (size_t)ipt;
+    return R * 8 + R * 2 + (F + 2) * 4 + F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4;
+}
+
+size_t lds_join(uint32_t table_log2)
+{
+    size_t hash = (size_t(2) << table_log2) * 4;
+    size_t smj = 2 * kSortChunk * 8 + kJoinNT * 4;
+    size_t nl = 2 * 4096 * 4;
+    size_t m = hash > smj ? hash : smj;
+    return m > nl ? m : nl;
+}
+
+hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
+{
+    if (!n) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
+                       uint32_t max_bits, hipStream_t st)
+{
+    if (!ntiles) return hipSuccess;
+    size_t l = lds_hist(max_bits);
+    hipError_t e = set_lds((const void *)k_hist, l);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk_ad, uint32_t nblk_ad,
+                       const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
+                       uint32_t max_bits, hipStream_t st)
+{
+    if (!nsegs) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_a, dim3(nblk_ad), dim3(256), 0, st, segs, blk_ad, counts, part);
+    hipLaunchKernelGGL(k_scan_b, dim3(nblk_b), dim3(256), 0, st, segs, blk_b, part, pstart);
+    size_t l = (size_t(1) << max_bits) * 4;
+    hipError_t e = set_lds((const void *)k_scan_c, l);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_c, dim3(nsegs), dim3(1024), l, st, segs, pstart);
+    hipLaunchKernelGGL(k_scan_d, dim3(nblk_ad), dim3(256), 0, st, segs, blk_ad, counts, part, pstart);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
+                          uint32_t *out_count, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st)
+{
+    if (!ntiles) return hipSuccess;
+    size_t l = lds_scatter(max_bits, ipt);
+    hipError_t e;
+    if (ipt == 16) {
+        e = set_lds((const void *)k_scatter<16>, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter<16>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_count, err);
+    } else {
+        e = set_lds((const void *)k_scatter<8>, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter<8>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_count, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_heavy_detect(const InstDesc *insts, uint32_t inst, uint32_t nS, uint32_t sample_blocks,
+                               uint32_t min_count, uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *nheavy,
+                               hipStream_t st)
+{
+    uint32_t total_blocks = (nS + kSampleBlock - 1) / kSampleBlock;
+    if (!total_blocks) return hipSuccess;
+    uint32_t stride = total_blocks / sample_blocks;
+    if (stride < 1) stride = 1;
+    uint32_t grid = (total_blocks + stride - 1) / stride;
+    size_t l = 2 * kSampleSlots * 4;
+    hipError_t e = set_lds((const void *)k_heavy_sample, l);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_heavy_sample, dim3(grid), dim3(512), l, st, insts, inst, stride, cand_keys, cand_cnt);
+    hipLaunchKernelGGL(k_heavy_select, dim3(1), dim3(1024), 0, st, insts, inst, cand_keys, cand_cnt, min_count, nheavy);
+    return hipGetLastError();
+}
+
+uint32_t cand_slots() { return kCandSlots; }
+
+hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
+                             WorkItem *items, uint32_t *nitems, hipStream_t st)
+{
+    if (!ninst) return hipSuccess;
+    hipLaunchKernelGGL(k_make_items, dim3(ninst), dim3(1024), 0, st, insts, pstart, smax, items, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, const uint32_t *nitems,
+                       uint32_t *qhead, uint32_t *out_count, uint32_t *err, uint32_t max_table_log2, uint32_t grid,
+                       hipStream_t st)
+{
+    size_t l = lds_join(max_table_log2);
+    hipError_t e = set_lds((const void *)k_join, l);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, pstart, items, nitems, qhead, out_count, err,
+                       max_table_log2);
+    return hipGetLastError();
+}
+
+}  // namespace spj
