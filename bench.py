@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark: the radix-partitioned equi-join on one MI355X.
+
+Default workload (one step = one full join on HBM-resident inputs):
+  |R| = 2^26 unique uint32 keys (permutation of 1..2^26, rid = row),
+  |S| = 2^27 keys drawn Zipf 0.5 over R's keys, hash join.
+Prints one JSON line: joined tuples/s = (|R| + |S|) / step time, plus the
+HBM roofline of the dominant kernel and a CPU baseline (the CPU restatement
+of the reference join on a bounded sample, on the host cores).
+
+--extra adds the skewed 2^26 x 2^28 runs (Zipf 0 / 1.0 / 1.5) and the
+sort-merge variant of the default workload to the same line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spid-join_amd"))
+
+import spidjoin as sj  # noqa: E402
+
+HBM_PEAK = 8.0e12   # MI355X HBM3E, B/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
+    """Generate, upload, warm up, then time `steps` joins.  Returns a dict."""
+    t0 = time.time()
+    if keep is not None and keep.get("key") == (nR, seed):
+        R = keep["R"]
+        dR = keep["dR"]
+    else:
+        if keep is not None and keep.get("dR") is not None:
+            keep["dR"].free()
+        R = sj.gen_r(nR, seed)
+        dR = sj.DeviceArray.from_numpy(R)
+        if keep is not None:
+            keep.update(key=(nR, seed), R=R, dR=dR)
+    S = sj.gen_s(nS, R, zipf, seed + 1)
+    dS = sj.DeviceArray.from_numpy(S)
+    del S
+    dO = sj.DeviceArray(nS * 8)
+    log(f"[bench] nR={nR} nS={nS} zipf={zipf}: data ready in {time.time() - t0:.1f}s")
+    try:
+        for _ in range(warmup):
+            n, _t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=False)
+            if n != nS:
+                raise RuntimeError(f"join produced {n} pairs, expected {nS}")
+        phases = []
+        sj.check(sj.lib().spj_dev_sync())
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            n, t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)
+            phases.append(t)
+        sj.check(sj.lib().spj_dev_sync())
+        wall = time.perf_counter() - t1
+        if n != nS:
+            raise RuntimeError(f"join produced {n} pairs, expected {nS}")
+    finally:
+        dS.free()
+        dO.free()
+    ms = wall / steps * 1e3
+    avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    log(f"[bench]   {ms:.3f} ms/step; phases {json.dumps({k: round(v, 4) for k, v in avg.items()})}")
+    return {"nR": nR, "nS": nS, "zipf": zipf, "ms_per_step": ms, "phases": avg, "pairs": int(n)}
+
+
+def roofline(res):
+    """Dominant kernel and its achieved algorithmic GB/s.
+
+    Algorithmic bytes per launch (one launch per phase per join):
+      k_hist    reads every tuple once:                 8 (|R| + |S|)
+      k_scatter reads every tuple, writes it to its
+                partition (or a heavy pair to out):     16 (|R| + |S|)
+      k_join    reads the partitioned R and S, writes
+                one pair per partitioned S tuple:       8 (|R'| + |S'|) + 8 |S'|
+    (|R'|, |S'|: tuples that went through partitions; M = |S| here.)"""
+    p = res["phases"]
+    nR, nS = res["nR"], res["nS"]
+    rp, spn = p["r_partitioned"], p["s_partitioned"]
+    kern = {
+        "k_hist": (p["hist_ms"], 8 * (nR + nS)),
+        "k_scatter": (p["scatter_ms"], 16 * (nR + nS)),
+        "k_join": (p["join_ms"], 8 * (rp + spn) + 8 * spn),
+    }
+    name = max(kern, key=lambda k: kern[k][0])
+    ms, byts = kern[name]
+    ach = byts / (ms * 1e-3)
+    b_alg = 24 * (nR + nS) + 8 * res["pairs"]
+    return name, {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": round(ach / HBM_PEAK, 4), "traffic": None}, {
+        k: {"ms": round(v[0], 4), "alg_bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)}
+        for k, v in kern.items()}, b_alg
+
+
+def cpu_baseline(zipf, seconds=10.0):
+    """The oracle's CPU restatement of the reference join (global SFH partition
+    into 1024 virtual DPUs, P_loc local partitions, 4096-slot linear probing)
+    on a bounded sample of the same workload, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    nR, nS = 1 << 22, 1 << 23   # 1/16 of the default workload, same R:S ratio and skew
+    R = sj.gen_r(nR, 7)
+    S = sj.gen_s(nS, R, zipf, 8)
+    oracle.spid_join_cpu(R, S, threads=threads)   # warm-up
+    ts = []
+    t_end = time.time() + seconds
+    while time.time() < t_end or len(ts) < 3:
+        t = time.perf_counter()
+        out = oracle.spid_join_cpu(R, S, threads=threads)
+        ts.append(time.perf_counter() - t)
+        if len(ts) >= 15:
+            break
+    if len(out) != nS:
+        raise RuntimeError("CPU baseline produced a wrong count")
+    med = float(np.median(ts))
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round((nR + nS) / med, 1), "unit": "joined tuples/s", "cores": threads, "kind": "port",
+            "sample": f"|R|=2^22, |S|=2^23, Zipf {zipf}, median of {len(ts)} runs ({med * 1e3:.1f} ms each); {cpu}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--extra", action="store_true", help="also run the skewed 2^26 x 2^28 and sort-merge runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    sj.init(0)
+    info = sj.device_info()
+    log(f"[bench] device {info}")
+    nR, nS, zipf = 1 << 26, 1 << 27, 0.5
+    keep = {}
+    res = run_workload(nR, nS, zipf, args.steps, args.warmup, None, 1000, keep)
+    kname, roof, kernels, b_alg = roofline(res)
+    ms = res["ms_per_step"]
+    value = (nR + nS) / (ms * 1e-3)
+    extra = {"kernels": kernels, "dominant_kernel": kname,
+             "join_B_alg_bytes": b_alg,
+             "join_roofline_frac": round(b_alg / (ms * 1e-3) / HBM_PEAK, 4),
+             "phases_ms": {k: round(v, 4) for k, v in res["phases"].items()}}
+    if args.extra:
+        smj = run_workload(nR, nS, zipf, max(3, args.steps // 4), 1, sj.config(algorithm=sj.ALG_SMJ), 1000, keep)
+        extra["smj"] = {"ms_per_step": round(smj["ms_per_step"], 4),
+                        "tuples_per_s": round((nR + nS) / (smj["ms_per_step"] * 1e-3), 1)}
+        sk = {}
+        for z in (0.0, 1.0, 1.5):
+            r = run_workload(1 << 26, 1 << 28, z, max(3, args.steps // 4), 1, None, 2000, keep)
+            bal = 24 * ((1 << 26) + (1 << 28)) + 8 * r["pairs"]
+            sk[str(z)] = {"ms_per_step": round(r["ms_per_step"], 4),
+                          "tuples_per_s": round(((1 << 26) + (1 << 28)) / (r["ms_per_step"] * 1e-3), 1),
+                          "roofline_frac": round(bal / (r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
+                          "heavy_keys": r["phases"]["heavy_keys"]}
+        sk["zipf1.5_over_zipf0"] = round(sk["1.5"]["ms_per_step"] / sk["0.0"]["ms_per_step"], 3)
+        extra["skewed_2^26x2^28"] = sk
+    if keep.get("dR") is not None:
+        keep["dR"].free()
+    cpu = None if args.no_cpu else cpu_baseline(zipf)
+    out = {
+        "metric": "joined tuples/s (|R|+|S|)",
+        "value": round(value, 1),
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "uint32",
+        "data": "synthetic (seeded generator: R = permutation of 1..2^26, S Zipf 0.5 over R's keys)",
+        "config": {"workload": "hash equi-join |R|=2^26 x |S|=2^27, Zipf 0.5, inputs resident in HBM",
+                   "nR": nR, "nS": nS, "zipf": zipf, "algorithm": "radix-partitioned hash join"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -49,7 +49,7 @@ enum { SPJ_ALG_AUTO = -1, SPJ_ALG_PHJ = 0, SPJ_ALG_NPHJ = 1, SPJ_ALG_SMJ = 2, SP
 typedef struct {
     int32_t algorithm;        /* SPJ_ALG_*: AUTO = hash, nested loop for tiny partitions */
     int32_t radix_bits;       /* partition-pass radix bits (0..12) */
-    int32_t table_log2;       /* LDS hash-table slots, log2 (6..14) */
+    int32_t table_log2;       /* LDS hash-table buckets, log2 (6..12; 12 = the reference's 4096) */
     int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
     int32_t scatter_ipt;      /* scatter tuples per thread per round: 8 or 16 */
@@ -57,8 +57,13 @@ typedef struct {
     int64_t items_s_max;      /* S tuples per join work item */
 } spj_config_t;
 
+/* Per-phase device time of the last call (HIP events on its stream) and, for
+ * single-instance calls, how many tuples went through the partitions (heavy
+ * S tuples are joined in the scatter and never partitioned). */
 typedef struct {
     float heavy_ms, hist_ms, scan_ms, scatter_ms, items_ms, join_ms, total_ms;
+    uint32_t heavy_keys;
+    uint64_t r_partitioned, s_partitioned;
 } spj_phase_times_t;
 
 void spj_config_default(spj_config_t *cfg);
@@ -76,6 +81,9 @@ int spj_dev_free(void *ptr);
 int spj_h2d(void *dst, const void *src, uint64_t bytes);
 int spj_d2h(void *dst, const void *src, uint64_t bytes);
 int spj_dev_memset(void *ptr, int value, uint64_t bytes);
+/* Device-to-device copy; returns the copy's device time in *ms when non-NULL
+ * (used to calibrate achievable HBM bandwidth). */
+int spj_d2d(void *dst, const void *src, uint64_t bytes, float *ms);
 int spj_dev_sync(void);
 
 /* One join on device-resident relations: the whole plan of

@@ -62,7 +62,7 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
 int check_resolved(const Resolved &r)
 {
     if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
-    if (r.table_log2 < 6 || r.table_log2 > 14) return fail(SPJ_EINVAL, "table_log2 %u outside 6..14", r.table_log2);
+    if (r.table_log2 < 6 || r.table_log2 > 12) return fail(SPJ_EINVAL, "table_log2 %u outside 6..12", r.table_log2);
     if (!(r.alg == 0 || r.alg == 2 || r.alg == 3)) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
     if (!(r.ipt == 8 || r.ipt == 16)) return fail(SPJ_EINVAL, "scatter_ipt %u (8 or 16)", r.ipt);
     if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);
@@ -91,6 +91,9 @@ void fill_times(const PhaseTimes &t, spj_phase_times_t *o)
     o->items_ms = t.items;
     o->join_ms = t.join;
     o->total_ms = t.total;
+    o->heavy_keys = t.heavy_keys;
+    o->r_partitioned = t.r_parted;
+    o->s_partitioned = t.s_parted;
 }
 
 int validate_sizes(uint64_t nR, uint64_t nS)
@@ -176,6 +179,21 @@ int spj_dev_memset(void *p, int v, uint64_t bytes)
 {
     hipError_t e = hipMemset(p, v, bytes);
     return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemset");
+}
+
+int spj_d2d(void *dst, const void *src, uint64_t bytes, float *ms)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, 0);
+    if (e == hipSuccess) e = hipEventRecord(b, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess && ms) e = hipEventElapsedTime(ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemcpy D2D");
 }
 
 int spj_dev_sync(void)

@@ -75,9 +75,10 @@ static uint32_t ceil_log2(uint64_t x)
 Resolved default_params(uint32_t nR, uint32_t nS)
 {
     Resolved r;
-    r.table_log2 = 14;
-    // Aim at ~8K R tuples per partition: half of a 16K-slot LDS table.
-    uint32_t b = nR > 8192 ? ceil_log2((uint64_t(nR) + 8191) / 8192) : 0;
+    r.table_log2 = 12;   // 4096 buckets, the reference table size
+    // Aim at <= 16K R tuples per partition: one LDS table (17408 tuples) with
+    // room for the binomial spread of partition sizes.
+    uint32_t b = nR > 16384 ? ceil_log2((uint64_t(nR) + 16383) / 16384) : 0;
     r.bits = std::min<uint32_t>(b, kMaxBits);
     r.ipt = 16;
     // Tiles: at least one scatter round; at most 2^24 counters per relation.
@@ -103,15 +104,19 @@ Context::~Context()
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
-    DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_,
-                      &rp_, &sp_, &items_, &ctrl_, &heavy_, &cand_};
+    DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_, &rp_,
+                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_};
     for (auto *b : bufs) b->release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// ctrl layout (uint32): [0] nitems, [1] qhead, [2..3] pad,
-// [4 .. 4+n) out_count, [4+n .. 4+2n) err, [4+2n .. 4+3n) nheavy
-static inline size_t ctrl_words(size_t n) { return 4 + 3 * n; }
+// ctrl layout (uint32): [0] nitems, [1] qhead, [2] holes recorded, [3] pad,
+// then n words each of: out_lo (pairs reserved by join items, from the front),
+// out_hi (heavy pairs, from the back), out_fill (pairs written by items),
+// err, nheavy.
+enum { kCtrlLo = 0, kCtrlHi = 1, kCtrlFill = 2, kCtrlErr = 3, kCtrlHeavy = 4, kCtrlArrays = 5 };
+static inline size_t ctrl_words(size_t n) { return 4 + kCtrlArrays * n; }
+static inline size_t ctrl_at(size_t n, int which, size_t i) { return 4 + (size_t)which * n + i; }
 
 hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed)
 {
@@ -207,6 +212,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     SPJ_TRY(rp_.ensure(rp_total * 8));
     SPJ_TRY(sp_.ensure(sp_total * 8));
     SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
+    SPJ_TRY(holes_.ensure(items_bound * sizeof(uint4)));
+    max_holes_ = (uint32_t)std::min<uint64_t>(items_bound, 0xFFFFFFFFull);
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
     if (nheavy_inst) {
         SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
@@ -233,7 +240,9 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
 
     const InstDesc *d_insts = insts_.as<InstDesc>();
     uint32_t *ctrl = ctrl_.as<uint32_t>();
-    uint32_t *out_count = ctrl + 4, *err = ctrl + 4 + n, *nheavy = ctrl + 4 + 2 * n;
+    uint32_t *out_lo = ctrl + ctrl_at(n, kCtrlLo, 0), *out_hi = ctrl + ctrl_at(n, kCtrlHi, 0);
+    uint32_t *out_fill = ctrl + ctrl_at(n, kCtrlFill, 0), *err = ctrl + ctrl_at(n, kCtrlErr, 0);
+    uint32_t *nheavy = ctrl + ctrl_at(n, kCtrlHeavy, 0);
     if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
     SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
 
@@ -257,7 +266,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
                         blk_b_.as<ScanBlock>(), (uint32_t)h_b_.size(), counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
-    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_count, err, max_bits_,
+    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
                            max_ipt_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
@@ -265,11 +274,47 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), smax, items_.as<WorkItem>(), ctrl, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
     const DeviceInfo &di = device_info();
-    uint32_t grid = (uint32_t)std::max(1, di.cus) * ((max_table_ <= 13) ? 2u : 1u);
-    SPJ_TRY(launch_join(d_insts, pstart_.as<uint32_t>(), items_.as<WorkItem>(), ctrl, ctrl + 1, out_count, err,
-                        max_table_, grid, st));
+    const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
+    SPJ_TRY(launch_join(d_insts, pstart_.as<uint32_t>(), items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
+                        holes_.as<uint4>(), max_holes_, grid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
+    specs_n_ = n;
+    h_ns_.resize(n);
+    for (size_t i = 0; i < n; i++) h_ns_[i] = specs[i].nS;
     return hipSuccess;
+}
+
+// Make instance i's output dense when it is not already: remove the holes
+// that items with unmatched S tuples left in [0, lo) and move the heavy pairs
+// from [nS - hi, nS) down behind the item pairs.  Only runs when some S tuple
+// found no R partner (never in a PK-FK join whose S keys all occur in R).
+hipError_t Context::compact(size_t i, uint32_t lo, uint32_t hi, uint32_t fill, const std::vector<uint4> &holes)
+{
+    const InstDesc &d = h_insts_[i];
+    std::vector<uint4> hs;
+    for (const uint4 &h : holes)
+        if (h.x == i) hs.push_back(h);
+    std::sort(hs.begin(), hs.end(), [](const uint4 &a, const uint4 &b) { return a.y < b.y; });
+    std::vector<uint4> segs;   // (src, dst, len)
+    uint32_t pos = 0, dst = 0;
+    for (const uint4 &h : hs) {
+        if (h.y > pos) segs.push_back(make_uint4(pos, dst, h.y - pos, 0));
+        dst += h.y > pos ? h.y - pos : 0;
+        pos = h.y + h.z;
+    }
+    if (lo > pos) {
+        segs.push_back(make_uint4(pos, dst, lo - pos, 0));
+        dst += lo - pos;
+    }
+    if (dst != fill) return hipErrorUnknown;   // hole records incomplete
+    if (hi) segs.push_back(make_uint4(d.nS - hi, fill, hi, 0));
+    const uint64_t total = (uint64_t)fill + hi;
+    SPJ_TRY(tmp_.ensure(total * 8 + 8));
+    SPJ_TRY(segbuf_.ensure(segs.size() * sizeof(uint4) + 16));
+    SPJ_TRY(hipMemcpyAsync(segbuf_.p, segs.data(), segs.size() * sizeof(uint4), hipMemcpyHostToDevice, stream_));
+    SPJ_TRY(launch_gather_segments(d.out, tmp_.as<uint2>(), segbuf_.as<uint4>(), (uint32_t)segs.size(), stream_));
+    SPJ_TRY(hipMemcpyAsync(d.out, tmp_.p, total * 8, hipMemcpyDeviceToDevice, stream_));
+    return hipStreamSynchronize(stream_);
 }
 
 hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed)
@@ -277,8 +322,28 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
     h_ctrl_.resize(ctrl_words(n));
     SPJ_TRY(hipMemcpyAsync(h_ctrl_.data(), ctrl_.p, ctrl_words(n) * 4, hipMemcpyDeviceToHost, stream_));
     SPJ_TRY(hipStreamSynchronize(stream_));
-    counts.assign(h_ctrl_.begin() + 4, h_ctrl_.begin() + 4 + n);
-    errs.assign(h_ctrl_.begin() + 4 + n, h_ctrl_.begin() + 4 + 2 * n);
+    counts.resize(n);
+    errs.assign(h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0), h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0) + n);
+    std::vector<uint4> holes;
+    bool holes_read = false;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t lo = h_ctrl_[ctrl_at(n, kCtrlLo, i)], hi = h_ctrl_[ctrl_at(n, kCtrlHi, i)];
+        const uint32_t fill = h_ctrl_[ctrl_at(n, kCtrlFill, i)];
+        counts[i] = fill + hi;
+        const bool dense = (fill == lo) && (hi == 0 || (uint64_t)lo + hi == h_insts_[i].nS);
+        if (dense || errs[i]) continue;
+        if (fill != lo && !holes_read) {
+            const uint32_t nh = h_ctrl_[2];
+            if (nh > max_holes_) {
+                errs[i] |= kErrOutOverflow;
+                continue;
+            }
+            holes.resize(nh);
+            if (nh) SPJ_TRY(hipMemcpy(holes.data(), holes_.p, nh * sizeof(uint4), hipMemcpyDeviceToHost));
+            holes_read = true;
+        }
+        SPJ_TRY(compact(i, lo, hi, fill, holes));
+    }
     if (timed) {
         float ms[6];
         for (int i = 0; i < 6; i++) SPJ_TRY(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
@@ -289,6 +354,16 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         timed->items = ms[4];
         timed->join = ms[5];
         SPJ_TRY(hipEventElapsedTime(&timed->total, ev_[0], ev_[6]));
+        if (n) {
+            timed->heavy_keys = h_ctrl_[ctrl_at(n, kCtrlHeavy, 0)];
+            const InstDesc &d0 = h_insts_[0];
+            const uint32_t F = 1u << d0.bits;
+            uint32_t tr = 0, ts = 0;
+            SPJ_TRY(hipMemcpy(&tr, pstart_.as<uint32_t>() + d0.pstartR + F, 4, hipMemcpyDeviceToHost));
+            SPJ_TRY(hipMemcpy(&ts, pstart_.as<uint32_t>() + d0.pstartS + F, 4, hipMemcpyDeviceToHost));
+            timed->r_parted = tr;
+            timed->s_parted = ts;
+        }
     }
     return hipSuccess;
 }

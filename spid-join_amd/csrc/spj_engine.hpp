@@ -36,6 +36,8 @@ struct InstSpec {
 // Milliseconds per phase (HIP events on the context's stream).
 struct PhaseTimes {
     float heavy = 0, hist = 0, scan = 0, scatter = 0, items = 0, join = 0, total = 0;
+    uint32_t heavy_keys = 0;            // instance 0
+    uint64_t r_parted = 0, s_parted = 0;  // instance 0: tuples written to partitions
 };
 
 struct DevBuf {
@@ -80,7 +82,11 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_;
+    uint32_t max_holes_ = 0;
+    size_t specs_n_ = 0;
+    std::vector<uint32_t> h_ns_;
+    hipError_t compact(size_t inst, uint32_t lo, uint32_t hi, uint32_t fill, const std::vector<uint4> &holes);
     std::vector<InstDesc> h_insts_;
     std::vector<TileDesc> h_tiles_;
     std::vector<ScanSeg> h_segs_;

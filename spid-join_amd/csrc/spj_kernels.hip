@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t join_hash(const uint32_t *crc, uint32_t k)
 // Heavy-table slot: multiplicative hash (the table is this build's own).
 __device__ __forceinline__ uint32_t heavy_slot(uint32_t key)
 {
-    return (key * 0x9E3779B1u) >> (32 - 12);   // kHeavySlots = 4096
+    return (key * 0x9E3779B1u) >> (32 - 11);   // kHeavySlots = 2048
 }
 
 // Sub-table of a partition that needs Q > 1 LDS tables: high SFH bits, which
@@ -180,24 +180,24 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
     if (heavy) load_heavy(I, hk, hr);
     __syncthreads();
     const uint2 *in = td.rel ? I.S : I.R;
-    if (!heavy) {
-        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
-            uint2 v = in[i];
-            if (v.x != kEmptyKey) atomicAdd(&hist[sfh(v.x) & mask], 1u);
+    const uint32_t mode = !heavy ? 0u : (td.rel == 0 ? 1u : 2u);
+    constexpr uint32_t U = 8;   // loads in flight per thread
+    for (uint32_t b = td.begin; b < td.end; b += U * blockDim.x) {
+        uint2 v[U];
+#pragma unroll
+        for (uint32_t j = 0; j < U; j++) {
+            const uint32_t i = b + j * blockDim.x + threadIdx.x;
+            v[j] = (i < td.end) ? in[i] : make_uint2(kEmptyKey, 0u);
         }
-    } else if (td.rel == 0) {
-        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
-            uint2 v = in[i];
-            if (v.x == kEmptyKey) continue;
-            atomicAdd(&hist[sfh(v.x) & mask], 1u);
-            int s = heavy_find(hk, v.x);
-            if (s >= 0) I.heavy_rids[s] = v.y;   // R keys are unique: one writer per slot
-        }
-    } else {
-        for (uint32_t i = td.begin + threadIdx.x; i < td.end; i += blockDim.x) {
-            uint2 v = in[i];
-            if (v.x == kEmptyKey || heavy_find(hk, v.x) >= 0) continue;
-            atomicAdd(&hist[sfh(v.x) & mask], 1u);
+#pragma unroll
+        for (uint32_t j = 0; j < U; j++) {
+            if (v[j].x == kEmptyKey) continue;
+            if (mode == 2 && heavy_find(hk, v[j].x) >= 0) continue;   // heavy S tuple: no partition
+            atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
+            if (mode == 1) {
+                const int s = heavy_find(hk, v[j].x);
+                if (s >= 0) I.heavy_rids[s] = v[j].y;   // R keys are unique: one writer per slot
+            }
         }
     }
     __syncthreads();
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_scan_d(const ScanSeg *__restrict__ segs
 template <int IPT>
 __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
                                                  uint32_t ntiles, const uint32_t *__restrict__ counts,
-                                                 uint32_t *__restrict__ out_count, uint32_t *__restrict__ err)
+                                                 uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
 {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t NT = 512, ROUND = NT * IPT;
@@ -304,48 +304,68 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
     uint2 *stage = reinterpret_cast<uint2 *>(lds);                  // ROUND tuples
     uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
     uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // F + 2: counts -> starts
-    uint32_t *off = lst + F + 2;                                    // F running offsets
-    uint32_t *wtmp = off + F;                                       // NT/64 + 1
+    uint32_t *off = lst + F + 2;                                    // F running write offsets
+    uint32_t *bs = off + F;                                         // F: off[d] - start[d] this round
+    uint32_t *wtmp = bs + F;                                        // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
     uint32_t *hk = misc + 4;                                        // kHeavySlots
     uint32_t *hr = hk + kHeavySlots;
 
     const uint32_t *cnt = counts + td.cnt_off;
     for (uint32_t i = threadIdx.x; i < F; i += NT) off[i] = cnt[i];
+    for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
     if (heavy) load_heavy(I, hk, hr);
     const uint2 *in = td.rel ? I.S : I.R;
     uint2 *outp = td.rel ? I.Sp : I.Rp;
 
+    // software pipeline: the next round's tuples are loaded while this round
+    // is ranked, staged and written
+    uint2 nv[IPT];
+    {
+        const uint32_t rn = min(ROUND, td.end - td.begin);
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            const uint32_t li = j * NT + threadIdx.x;
+            if (li < rn) nv[j] = in[td.begin + li];
+        }
+    }
+    __syncthreads();
     for (uint32_t rb = td.begin; rb < td.end; rb += ROUND) {
         const uint32_t rn = min(ROUND, td.end - rb);
-        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-        __syncthreads();
         uint2 v[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) v[j] = nv[j];
+        if (rb + ROUND < td.end) {
+            const uint32_t nb = rb + ROUND, nn = min(ROUND, td.end - nb);
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                if (li < nn) nv[j] = in[nb + li];
+            }
+        }
         uint32_t dg[IPT], rk[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
             const uint32_t li = j * NT + threadIdx.x;
             dg[j] = DROP;
-            if (li < rn) {
-                v[j] = in[rb + li];
-                if (v[j].x != kEmptyKey) {
-                    if (heavy) {
-                        int s = heavy_find(hk, v[j].x);
-                        if (s >= 0) {
-                            // heavy: bucket F, unless the key has no R tuple
-                            if (hr[s] != kNoRid) { dg[j] = F; v[j].x = hr[s]; }
-                        } else {
-                            dg[j] = sfh(v[j].x) & mask;
-                        }
+            if (li < rn && v[j].x != kEmptyKey) {
+                if (heavy) {
+                    const int s = heavy_find(hk, v[j].x);
+                    if (s >= 0) {
+                        // heavy: bucket F, unless the key has no R tuple
+                        if (hr[s] != kNoRid) { dg[j] = F; v[j].x = hr[s]; }
                     } else {
                         dg[j] = sfh(v[j].x) & mask;
                     }
+                } else {
+                    dg[j] = sfh(v[j].x) & mask;
                 }
             }
             if (dg[j] != DROP) rk[j] = atomicAdd(&lst[dg[j]], 1u);
         }
         __syncthreads();
         const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
+        const uint32_t hstart = lst[F];                                  // == total without heavy tuples
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
             if (dg[j] != DROP) {
@@ -354,30 +374,27 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
                 sdig[p] = (uint16_t)dg[j];
             }
         }
-        const uint32_t hstart = heavy ? lst[F] : total;
+        for (uint32_t d = threadIdx.x; d < F; d += NT) {
+            const uint32_t s0 = lst[d], s1 = (d + 1 < F) ? lst[d + 1] : hstart;
+            bs[d] = off[d] - s0;
+            off[d] += s1 - s0;
+        }
         if (heavy && threadIdx.x == 0) {
+            // heavy pairs fill the output from its end downwards
             const uint32_t hn = total - hstart;
-            misc[0] = hn ? atomicAdd(&out_count[td.inst], hn) : 0u;
+            misc[0] = hn ? I.nS - atomicAdd(&out_hi[td.inst], hn) - hn : 0u;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < hstart; i += NT) {
-            const uint32_t d = sdig[i];
-            outp[off[d] + (i - lst[d])] = stage[i];
-        }
+        for (uint32_t i = threadIdx.x; i < hstart; i += NT) outp[bs[sdig[i]] + i] = stage[i];
         if (heavy) {
             const uint32_t hb = misc[0];
             for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
                 const uint32_t o = hb + (i - hstart);
-                const uint2 sv = stage[i];   // .x already holds the R rid
-                if (o < I.nS) I.out[o] = make_uint2(sv.x, sv.y);
+                if (o < I.nS) I.out[o] = stage[i];   // .x already holds the R rid
                 else atomicOr(&err[td.inst], kErrOutOverflow);
             }
         }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < F; d += NT) {
-            const uint32_t nx = (d + 1 < F) ? lst[d + 1] : hstart;
-            off[d] += nx - lst[d];
-        }
+        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         __syncthreads();
     }
 }
@@ -503,33 +520,41 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 // ---------------------------------------------------------------------------
 // k_join: persistent workgroups pop items from the queue.  Per item the
 // algorithm is chosen by partition size and the instance's setting:
-//   hash (a2a_srj_build.c + a2a_srj_probe_inner_mt.c): R_p in an LDS open-
-//     addressing table, slot = join_hash & (slots-1), linear probing, key 0 =
-//     empty; a partition larger than one table is split into Q sub-tables by
-//     high hash bits, each probed by the S tuples of its sub-range;
+//   hash (a2a_srj_build.c + a2a_srj_probe_inner_mt.c): R_p is counting-
+//     sorted into an LDS table by bucket = join_hash & 0xFFF -- the
+//     reference's 4096-slot table index -- so one bucket holds every key that
+//     would have probed from that slot; a probe scans one bucket (first equal
+//     key wins).  Load factor 1: up to kTableCap R tuples per workgroup.  A
+//     larger R_p is split into Q sub-tables by high hash bits;
 //   nested loop (nested_loop_join.c) for tiny R_p;
 //   sort-merge (mpsm_join_sort_probe.c): R_p and S chunks bitonic-sorted in
 //     LDS, then a per-thread merge.
-// Matches are counted per wave with ballots and appended with one global
-// atomic per round, so the output has exact counts and no padding.
+// Output: each item reserves |S range| pairs of its instance's output once
+// (items grow from the front; heavy pairs from the back), waves append with
+// one LDS atomic each, and an item that matched fewer tuples leaves a hole
+// that is recorded for the compaction step -- exact counts, no padding.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kJoinNT = 512;
+constexpr uint32_t kJoinNT = 1024;
 constexpr uint32_t kJoinIPT = 8;
 constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
+constexpr uint32_t kBuckets = 4096;         // join_hash & 0xFFF (a2a_srj_build.c:209-212)
+constexpr uint32_t kTableCap = 17408;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kSortChunk = 8192;       // sort-merge chunk (u64 words) per side
+constexpr uint32_t kNLChunk = 8192;
 
 struct JoinShared {
     uint32_t item;
-    uint32_t round_total;
-    uint32_t round_base;
-    uint32_t pad;
+    uint32_t fill;   // pairs written in the item's reservation
+    uint32_t base;   // reservation start
+    uint32_t res;    // reservation length
+    uint32_t ok;     // table built (no overflow)
 };
 
-// Emit the matches held by this thread for one round: rr[j]/sr[j] valid when
-// (hit >> j) & 1.  All threads of the block must call it.
+// Append this thread's hits of one round (bit j of hit: pair rr[j], sr[j]).
+// Positions come from per-instruction ballots; one LDS atomic per wave.
 template <int IPT>
-__device__ __forceinline__ void emit_round(const uint32_t *rr, const uint32_t *sr, uint32_t hit, JoinShared *sh,
-                                           uint32_t *out_count_i, uint2 *out, uint32_t cap, uint32_t *err_i)
+__device__ __forceinline__ void wave_append(const uint32_t *rr, const uint32_t *sr, uint32_t hit, JoinShared *sh,
+                                            uint2 *out, uint32_t *err_i)
 {
     uint64_t m[IPT];
     uint32_t wtot = 0;
@@ -538,22 +563,17 @@ __device__ __forceinline__ void emit_round(const uint32_t *rr, const uint32_t *s
         m[j] = __ballot((hit >> j) & 1u);
         wtot += __popcll(m[j]);
     }
-    uint32_t wbase = 0;
-    if (lane_id() == 0 && wtot) wbase = atomicAdd(&sh->round_total, wtot);
-    wbase = __shfl(wbase, 0, kWave);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t tot = sh->round_total;
-        sh->round_base = tot ? atomicAdd(out_count_i, tot) : 0u;
-        sh->round_total = 0;
-    }
-    __syncthreads();
-    uint32_t o = sh->round_base + wbase;
+    if (wtot == 0) return;
+    uint32_t wb = 0;
+    if (lane_id() == 0) wb = atomicAdd(&sh->fill, wtot);
+    wb = __shfl(wb, 0, kWave);
+    const uint32_t lim = sh->res;
+    uint32_t o = wb;
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         if ((hit >> j) & 1u) {
             const uint32_t pos = o + mbcnt(m[j]);
-            if (pos < cap) out[pos] = make_uint2(rr[j], sr[j]);
+            if (pos < lim) out[sh->base + pos] = make_uint2(rr[j], sr[j]);
             else atomicOr(err_i, kErrOutOverflow);
         }
         o += __popcll(m[j]);
@@ -577,10 +597,22 @@ __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* powe
     }
 }
 
+// Load one probe round: S tuples [s0, s0 + rn) of the item, IPT per thread.
+template <int IPT>
+__device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, uint32_t s0, uint32_t se)
+{
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t si = s0 + j * kJoinNT + threadIdx.x;
+        v[j] = (si < se) ? S[si] : make_uint2(kEmptyKey, 0u);
+    }
+}
+
 __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
-                                                  const WorkItem *__restrict__ items, const uint32_t *__restrict__ nitems,
-                                                  uint32_t *__restrict__ qhead, uint32_t *__restrict__ out_count,
-                                                  uint32_t *__restrict__ err, uint32_t max_table_log2)
+                                                  const WorkItem *__restrict__ items, uint32_t *__restrict__ ctrl,
+                                                  uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill,
+                                                  uint32_t *__restrict__ err, uint4 *__restrict__ holes,
+                                                  uint32_t max_holes)
 {
     extern __shared__ uint32_t lds[];
     __shared__ JoinShared sh;
@@ -588,11 +620,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
     __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
     constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
     build_crc_lds(crc);
-    if (threadIdx.x == 0) sh.round_total = 0;
-    const uint32_t total_items = *nitems;
+    const uint32_t total_items = ctrl[0];
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) sh.item = atomicAdd(qhead, 1u);
+        if (threadIdx.x == 0) sh.item = atomicAdd(&ctrl[1], 1u);
         __syncthreads();
         const uint32_t it = sh.item;
         if (it >= total_items) break;
@@ -602,39 +633,41 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         const uint32_t nr = re - rb;
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
-        uint32_t *ocnt = out_count + w.inst;
         uint32_t *erri = err + w.inst;
+        if (threadIdx.x == 0) {
+            sh.res = w.se - w.sb;
+            sh.base = atomicAdd(&out_lo[w.inst], sh.res);
+            sh.fill = 0;
+        }
+        __syncthreads();
 
         if (alg == 3) {
-            // ---------------- nested loop: R chunks in LDS, every S tuple
+            // ---------------- nested loop: R chunks in LDS; every S tuple
             // compares against every R key of the chunk (first match wins)
-            uint32_t *rk = lds, *rv = lds + 4096;
-            for (uint32_t c0 = 0; c0 < nr; c0 += 4096) {
-                const uint32_t cn = min(4096u, nr - c0);
+            uint32_t *rk = lds, *rv = lds + kNLChunk;
+            for (uint32_t c0 = 0; c0 < nr; c0 += kNLChunk) {
+                const uint32_t cn = min(kNLChunk, nr - c0);
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = I.Rp[rb + c0 + i]; rk[i] = r.x; rv[i] = r.y; }
                 __syncthreads();
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                    uint2 v[IPT];
+                    load_round<IPT>(v, I.Sp, s0, w.se);
                     uint32_t rr[IPT], sr[IPT], hit = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
-                        const uint32_t si = s0 + j * NT + threadIdx.x;
-                        if (si < w.se) {
-                            const uint2 sv = I.Sp[si];
-                            for (uint32_t i = 0; i < cn; i++) {
-                                if (rk[i] == sv.x) { rr[j] = rv[i]; sr[j] = sv.y; hit |= 1u << j; break; }
-                            }
+                        if (v[j].x == kEmptyKey) continue;
+                        for (uint32_t i = 0; i < cn; i++) {
+                            if (rk[i] == v[j].x) { rr[j] = rv[i]; sr[j] = v[j].y; hit |= 1u << j; break; }
                         }
                     }
-                    emit_round<IPT>(rr, sr, hit, &sh, ocnt, I.out, I.nS, erri);
+                    wave_append<IPT>(rr, sr, hit, &sh, I.out, erri);
                 }
             }
-            continue;
-        }
-
-        if (alg == 2) {
+        } else if (alg == 2) {
             // ---------------- sort-merge: R chunk sorted once, each S chunk
-            // sorted and merged against it (S tuple takes the equal R key's rid)
+            // sorted and merged against it (an S tuple takes the rid of the
+            // equal R key, mpsm_join_sort_probe.c:214-236)
             uint64_t *ra = reinterpret_cast<uint64_t *>(lds);
             uint64_t *sa = ra + kSortChunk;
             uint32_t *cnts = reinterpret_cast<uint32_t *>(sa + kSortChunk);   // NT
@@ -662,8 +695,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     // merge: thread owns S[a, b); start from lower_bound in R
                     const uint32_t per = (sn + NT - 1) / NT;
                     const uint32_t a = min(sn, threadIdx.x * per), b = min(sn, a + per);
-                    uint32_t c = 0;
-                    uint32_t r0 = 0;
+                    uint32_t c = 0, r0 = 0;
                     if (a < b) {
                         const uint32_t k0 = (uint32_t)(sa[a] >> 32);
                         uint32_t lo = 0, hi = cn;
@@ -673,83 +705,129 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         for (uint32_t i = a; i < b; i++) {
                             const uint32_t k = (uint32_t)(sa[i] >> 32);
                             while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
-                            if (r < cn && (uint32_t)(ra[r] >> 32) == k) c++;
+                            if (k != kEmptyKey && r < cn && (uint32_t)(ra[r] >> 32) == k) c++;
                         }
                     }
                     cnts[threadIdx.x] = c;
                     __syncthreads();
                     const uint32_t tot = block_exclusive_scan(cnts, NT, wtmp);
-                    if (threadIdx.x == 0) sh.round_base = tot ? atomicAdd(ocnt, tot) : 0u;
-                    __syncthreads();
-                    uint32_t o = sh.round_base + cnts[threadIdx.x];
+                    uint32_t o = sh.fill + cnts[threadIdx.x];
                     if (a < b) {
                         uint32_t r = r0;
                         for (uint32_t i = a; i < b; i++) {
                             const uint32_t k = (uint32_t)(sa[i] >> 32);
                             while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
-                            if (r < cn && (uint32_t)(ra[r] >> 32) == k) {
-                                if (o < I.nS) I.out[o] = make_uint2((uint32_t)ra[r], (uint32_t)sa[i]);
+                            if (k != kEmptyKey && r < cn && (uint32_t)(ra[r] >> 32) == k) {
+                                if (o < sh.res) I.out[sh.base + o] = make_uint2((uint32_t)ra[r], (uint32_t)sa[i]);
                                 else atomicOr(erri, kErrOutOverflow);
                                 o++;
                             }
                         }
                     }
                     __syncthreads();
+                    if (threadIdx.x == 0) sh.fill += tot;
+                    __syncthreads();
                 }
             }
-            continue;
-        }
-
-        // ---------------- hash join
-        const uint32_t maxslots = 1u << min(I.table_log2, max_table_log2);
-        const uint32_t cap = (maxslots / 4u) * 3u;
-        const uint32_t Q = (nr + cap - 1) / cap;
-        const uint32_t nsub = (nr + Q - 1) / Q;
-        uint32_t slots = 64;
-        while (slots < 2 * nsub && slots < maxslots) slots <<= 1;
-        const uint32_t smask = slots - 1u;
-        uint32_t *tk = lds, *tv = lds + slots;
-        for (uint32_t q = 0; q < Q; q++) {
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < slots; i += NT) tk[i] = kEmptyKey;
-            __syncthreads();
-            // build
-            for (uint32_t i = rb + threadIdx.x; i < re; i += NT) {
-                const uint2 r = I.Rp[i];
-                if (Q > 1 && sub_of(sfh(r.x), I.bits, Q) != q) continue;
-                uint32_t h = join_hash(crc, r.x) & smask;
-                uint32_t step = 0;
-                for (; step < slots; step++) {
-                    const uint32_t old = atomicCAS(&tk[h], kEmptyKey, r.x);
-                    if (old == kEmptyKey) { tv[h] = r.y; break; }
-                    if (old == r.x) { atomicOr(erri, kErrDuplicateKey); break; }
-                    h = (h + 1u) & smask;
-                }
-                if (step == slots) atomicOr(erri, kErrTableOverflow);
-            }
-            __syncthreads();
-            // probe
-            for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
-                uint32_t rr[IPT], sr[IPT], hit = 0;
+        } else {
+            // ---------------- hash join
+            uint32_t *tk = lds, *tv = lds + kTableCap, *be = lds + 2 * kTableCap;
+            const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
+            const uint32_t nb = 1u << min(I.table_log2, 12u), bm = nb - 1u;   // buckets
+            for (uint32_t q = 0; q < Q; q++) {
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < nb; i += NT) be[i] = 0;
+                __syncthreads();
+                // count pass: tuples per bucket
+                for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                    uint2 r[4];
 #pragma unroll
-                for (int j = 0; j < (int)IPT; j++) {
-                    const uint32_t si = s0 + j * NT + threadIdx.x;
-                    if (si < w.se) {
-                        const uint2 sv = I.Sp[si];
-                        if (Q == 1 || sub_of(sfh(sv.x), I.bits, Q) == q) {
-                            uint32_t h = join_hash(crc, sv.x) & smask;
-                            for (uint32_t step = 0; step < slots; step++) {
-                                const uint32_t k = tk[h];
-                                if (k == sv.x) { rr[j] = tv[h]; sr[j] = sv.y; hit |= 1u << j; break; }
-                                if (k == kEmptyKey) break;
-                                h = (h + 1u) & smask;
-                            }
-                        }
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t i = i0 + u * NT + threadIdx.x;
+                        r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (r[u].x == kEmptyKey) continue;
+                        if (Q > 1 && sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
+                        atomicAdd(&be[join_hash(crc, r[u].x) & bm], 1u);
                     }
                 }
-                emit_round<IPT>(rr, sr, hit, &sh, ocnt, I.out, I.nS, erri);
+                __syncthreads();
+                const uint32_t nq = block_exclusive_scan(be, nb, wtmp);   // be[b] = start of bucket b
+                if (nq > kTableCap) {
+                    if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
+                    continue;   // uniform: nq is the same in every thread
+                }
+                // fill pass: be[b] advances to the end of bucket b
+                for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                    uint2 r[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t i = i0 + u * NT + threadIdx.x;
+                        r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (r[u].x == kEmptyKey) continue;
+                        if (Q > 1 && sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
+                        const uint32_t p = atomicAdd(&be[join_hash(crc, r[u].x) & bm], 1u);
+                        tk[p] = r[u].x;
+                        tv[p] = r[u].y;
+                    }
+                }
+                __syncthreads();
+                // duplicate R keys (a2a_srj_build.c:265-271): pairwise within each bucket
+                for (uint32_t b = threadIdx.x; b < nb; b += NT) {
+                    const uint32_t s = b ? be[b - 1] : 0u, e = be[b];
+                    if (e - s > 64) continue;
+                    for (uint32_t i = s; i < e; i++)
+                        for (uint32_t k = i + 1; k < e; k++)
+                            if (tk[i] == tk[k]) atomicOr(erri, kErrDuplicateKey);
+                }
+                // probe, with the next round's loads in flight
+                uint2 nv[IPT];
+                load_round<IPT>(nv, I.Sp, w.sb, w.se);
+                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                    uint2 v[IPT];
+#pragma unroll
+                    for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
+                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, s0 + ROUND, w.se);
+                    uint32_t rr[IPT], sr[IPT], hit = 0;
+#pragma unroll
+                    for (int j = 0; j < (int)IPT; j++) {
+                        const uint32_t k = v[j].x;
+                        if (k == kEmptyKey) continue;
+                        if (Q > 1 && sub_of(sfh(k), I.bits, Q) != q) continue;
+                        const uint32_t b = join_hash(crc, k) & bm;
+                        const uint32_t e = be[b];
+                        for (uint32_t i = b ? be[b - 1] : 0u; i < e; i++) {
+                            if (tk[i] == k) { rr[j] = tv[i]; sr[j] = v[j].y; hit |= 1u << j; break; }
+                        }
+                    }
+                    wave_append<IPT>(rr, sr, hit, &sh, I.out, erri);
+                }
             }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (sh.fill) atomicAdd(&out_fill[w.inst], sh.fill);
+            if (sh.fill < sh.res) {
+                // hole [base + fill, base + res) for the compaction step
+                const uint32_t h = atomicAdd(&ctrl[2], 1u);
+                if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
+            }
+        }
+    }
+}
+
+// Copy segments of pairs: seg = (src offset, dst offset, length) in pairs.
+__global__ void k_gather_segments(const uint2 *__restrict__ src, uint2 *__restrict__ dst, const uint4 *__restrict__ segs,
+                                  uint32_t nsegs)
+{
+    for (uint32_t s = blockIdx.x; s < nsegs; s += gridDim.x) {
+        const uint4 g = segs[s];
+        for (uint32_t i = threadIdx.x; i < g.z; i += blockDim.x) dst[g.y + i] = src[g.x + i];
     }
 }
 
@@ -767,17 +845,19 @@ size_t lds_hist(uint32_t max_bits) { return ((size_t(1) << max_bits) + 2 * kHeav
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = 512u * (size_t)ipt;
-    return R * 8 + R * 2 + (F + 2) * 4 + F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4;
+    return R * 8 + R * 2 + (F + 2) * 4 + 2 * F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4;
 }
 
-size_t lds_join(uint32_t table_log2)
+size_t lds_join()
 {
-    size_t hash = (size_t(2) << table_log2) * 4;
+    size_t hash = (2 * (size_t)kTableCap + kBuckets) * 4;
     size_t smj = 2 * kSortChunk * 8 + kJoinNT * 4;
-    size_t nl = 2 * 4096 * 4;
+    size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
     return m > nl ? m : nl;
 }
+
+uint32_t join_table_cap() { return kTableCap; }
 
 hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
 {
@@ -815,7 +895,7 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
 }
 
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                          uint32_t *out_count, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st)
+                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st)
 {
     if (!ntiles) return hipSuccess;
     size_t l = lds_scatter(max_bits, ipt);
@@ -823,11 +903,11 @@ hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t
     if (ipt == 16) {
         e = set_lds((const void *)k_scatter<16>, l);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_scatter<16>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_count, err);
+        hipLaunchKernelGGL(k_scatter<16>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_hi, err);
     } else {
         e = set_lds((const void *)k_scatter<8>, l);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_scatter<8>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_count, err);
+        hipLaunchKernelGGL(k_scatter<8>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_hi, err);
     }
     return hipGetLastError();
 }
@@ -859,15 +939,23 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, const uint32_t *nitems,
-                       uint32_t *qhead, uint32_t *out_count, uint32_t *err, uint32_t max_table_log2, uint32_t grid,
-                       hipStream_t st)
+hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, uint32_t *ctrl,
+                       uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
+                       uint32_t grid, hipStream_t st)
 {
-    size_t l = lds_join(max_table_log2);
+    size_t l = lds_join();
     hipError_t e = set_lds((const void *)k_join, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, pstart, items, nitems, qhead, out_count, err,
-                       max_table_log2);
+    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, pstart, items, ctrl, out_lo, out_fill, err,
+                       holes, max_holes);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *segs, uint32_t nsegs, hipStream_t st)
+{
+    if (!nsegs) return hipSuccess;
+    uint32_t grid = nsegs < 4096 ? nsegs : 4096;
+    hipLaunchKernelGGL(k_gather_segments, dim3(grid), dim3(256), 0, st, src, dst, segs, nsegs);
     return hipGetLastError();
 }
 

@@ -10,7 +10,8 @@ namespace spj {
 
 size_t lds_hist(uint32_t max_bits);
 size_t lds_scatter(uint32_t max_bits, int ipt);
-size_t lds_join(uint32_t table_log2);
+size_t lds_join();
+uint32_t join_table_cap();
 uint32_t cand_slots();
 
 hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -20,14 +21,15 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
                        const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
                        uint32_t max_bits, hipStream_t st);
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                          uint32_t *out_count, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st);
+                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st);
 hipError_t launch_heavy_detect(const InstDesc *insts, uint32_t inst, uint32_t nS, uint32_t sample_blocks,
                                uint32_t min_count, uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *nheavy,
                                hipStream_t st);
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
                              WorkItem *items, uint32_t *nitems, hipStream_t st);
-hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, const uint32_t *nitems,
-                       uint32_t *qhead, uint32_t *out_count, uint32_t *err, uint32_t max_table_log2, uint32_t grid,
-                       hipStream_t st);
+hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, uint32_t *ctrl,
+                       uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
+                       uint32_t grid, hipStream_t st);
+hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *segs, uint32_t nsegs, hipStream_t st);
 
 }  // namespace spj

@@ -23,7 +23,7 @@ ERRORS = {0: "OK", -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EDUPKEY", -5: "E
 # Every function the C header declares (checked by the CPU tests).
 EXPORTS = [
     "spj_config_default", "spj_last_error", "spj_init", "spj_device_info", "spj_dev_alloc", "spj_dev_free",
-    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_dev_sync", "spj_join_device", "spj_join_host",
+    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_dev_sync", "spj_join_device", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_free_result", "spj_destroy",
     "spj_tuner_set", "spj_tuner_get", "spj_tuner_clear", "spj_tuner_entries", "spj_tuner_save",
@@ -45,10 +45,12 @@ class Config(ctypes.Structure):
 
 class PhaseTimes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in
-                ("heavy_ms", "hist_ms", "scan_ms", "scatter_ms", "items_ms", "join_ms", "total_ms")]
+                ("heavy_ms", "hist_ms", "scan_ms", "scatter_ms", "items_ms", "join_ms", "total_ms")] + [
+        ("heavy_keys", ctypes.c_uint32), ("r_partitioned", ctypes.c_uint64), ("s_partitioned", ctypes.c_uint64)]
 
     def as_dict(self):
-        return {n: float(getattr(self, n)) for n, _ in self._fields_}
+        return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))
+                for n, t in self._fields_}
 
 
 class JoinDesign(ctypes.Structure):
@@ -82,6 +84,7 @@ def lib():
         "spj_d2h": (ctypes.c_int, [P, P, U64]),
         "spj_dev_memset": (ctypes.c_int, [P, ctypes.c_int, U64]),
         "spj_dev_sync": (ctypes.c_int, []),
+        "spj_d2d": (ctypes.c_int, [P, P, U64, ctypes.POINTER(ctypes.c_float)]),
         "spj_join_device": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config),
                                            ctypes.POINTER(PhaseTimes)]),
         "spj_join_host": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config)]),

@@ -1,4 +1,5 @@
-"""GPU parity: the HIP join through the C ABI against the oracle."""
+"""GPU parity: the HIP join through the C ABI against the oracle (bit-exact
+sorted multiset of (R rid, S rid) pairs, exact count)."""
 import numpy as np
 import pytest
 
@@ -11,16 +12,117 @@ def words(p):
 
 def check_equal(sj, orc, R, S, pairs):
     exp = orc.expected_pairs(R, S)
-    assert len(pairs) == len(exp)
+    assert len(pairs) == len(exp), (len(pairs), len(exp))
     assert np.array_equal(sj.sort_pairs(pairs), words(exp))
 
 
 @pytest.mark.parametrize("nR,ratio,zipf", [(1 << 12, 2, 0.0), (1 << 16, 4, 0.5), (1 << 20, 2, 0.0),
-                                           (1 << 18, 8, 1.5)])
+                                           (1 << 18, 8, 1.5), (1 << 20, 4, 1.0), (100_003, 3, 0.25)])
 def test_hash_join_matches_oracle(gpu, orc, nR, ratio, zipf):
     sj = gpu
     R = sj.gen_r(nR, 100 + nR)
     S = sj.gen_s(nR * ratio, R, zipf, 200 + nR)
+    print(f"case nR={nR} nS={len(S)} zipf={zipf}", flush=True)
     pairs, t = sj.join(R, S, timed=True)
-    print(f"nR={nR} nS={len(S)} zipf={zipf} times={t}", flush=True)
+    print(f"  times={t}", flush=True)
     check_equal(sj, orc, R, S, pairs)
+
+
+@pytest.mark.parametrize("alg", [2, 3])
+def test_sort_merge_and_nested_loop(gpu, orc, alg):
+    sj = gpu
+    nR = (1 << 16) if alg == 2 else 3000
+    R = sj.gen_r(nR, 7 + alg)
+    S = sj.gen_s(nR * 2, R, 0.8, 9 + alg)
+    print(f"case alg={alg} nR={nR}", flush=True)
+    pairs, _ = sj.join(R, S, sj.config(algorithm=alg))
+    check_equal(sj, orc, R, S, pairs)
+
+
+def test_unmatched_and_zero_keys(gpu, orc):
+    """S keys missing from R, key-0 tuples on both sides (skipped, as the
+    reference treats key 0 as empty), heavy keys without an R partner."""
+    sj = gpu
+    rng = np.random.default_rng(5)
+    nR, nS = 1 << 18, 1 << 21
+    R = sj.gen_r(nR, 21)
+    S = sj.gen_s(nS, R, 1.2, 22)
+    # drop every 3rd R tuple's key to 0 and shift some S keys out of range
+    R["key"][::3] = 0
+    miss = rng.random(nS) < 0.1
+    S["key"][miss] = S["key"][miss] + np.uint32(1 << 24)
+    S["key"][rng.random(nS) < 0.01] = 0
+    print("case unmatched", flush=True)
+    for heavy in (0, 1):
+        pairs, _ = sj.join(R, S, sj.config(heavy=heavy))
+        check_equal(sj, orc, R, S, pairs)
+
+
+def test_tiny_and_empty(gpu, orc):
+    sj = gpu
+    R = sj.gen_r(5, 1)
+    S = sj.gen_s(17, R, 0.0, 2)
+    pairs, _ = sj.join(R, S)
+    check_equal(sj, orc, R, S, pairs)
+    E = np.zeros(0, dtype=sj.TUPLE)
+    pairs, _ = sj.join(E, S)
+    assert len(pairs) == 0
+    pairs, _ = sj.join(R, E)
+    assert len(pairs) == 0
+
+
+def test_duplicate_r_key_is_an_error(gpu):
+    sj = gpu
+    R = sj.gen_r(1 << 12, 3)
+    R["key"][10] = R["key"][11]
+    S = sj.gen_s(1 << 13, sj.gen_r(1 << 12, 3), 0.0, 4)
+    with pytest.raises(sj.SpjError) as e:
+        sj.join(R, S)
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("bits", [0, 3, 9, 12])
+def test_radix_bits_sweep(gpu, orc, bits):
+    """Any radix fan-out gives the same multiset (large partitions are split
+    into several LDS tables)."""
+    sj = gpu
+    R = sj.gen_r(1 << 17, 31)
+    S = sj.gen_s(1 << 18, R, 0.5, 32)
+    print(f"case bits={bits}", flush=True)
+    pairs, _ = sj.join(R, S, sj.config(radix_bits=bits))
+    check_equal(sj, orc, R, S, pairs)
+
+
+def test_batch_matches_single(gpu, orc):
+    sj = gpu
+    rng = np.random.default_rng(9)
+    Rs, Ss = [], []
+    for i in range(40):
+        nR = int(2 ** rng.uniform(10, 16))
+        ratio = int(rng.choice([1, 2, 4, 8]))
+        R = sj.gen_r(nR, 1000 + i)
+        S = sj.gen_s(nR * ratio, R, float(rng.uniform(0, 1.25)), 2000 + i)
+        Rs.append(R)
+        Ss.append(S)
+    outs, _ = sj.join_batch(Rs, Ss)
+    for R, S, p in zip(Rs, Ss, outs):
+        check_equal(sj, orc, R, S, p)
+
+
+def test_join_instance_lifecycle(gpu, orc):
+    """LoadColumn / PrepareJoin / StartJoin (async) / FinishJoin, query.cc."""
+    sj = gpu
+    R = sj.gen_r(1 << 16, 41)
+    S = sj.gen_s(1 << 18, R, 1.0, 42)
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    d = sj.JoinInstance.design(join_alg=0, bank_set=8)
+    ji.prepare(d)
+    ji.start(d)
+    pairs = ji.finish()
+    check_equal(sj, orc, R, S, pairs)
+    assert ji.phase_times()["total_ms"] > 0
+    with pytest.raises(sj.SpjError):
+        ji.finish()   # FinishJoin without StartJoin
+    ji.close()
