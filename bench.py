@@ -137,6 +137,22 @@ def cpu_baseline(zipf, seconds=10.0):
             "sample": f"|R|=2^22, |S|=2^23, Zipf {zipf}, median of {len(ts)} runs ({med * 1e3:.1f} ms each); {cpu}"}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of nbytes: each reads nbytes and writes nbytes."""
+    import ctypes
+    a = sj.DeviceArray(nbytes)
+    b = sj.DeviceArray(nbytes)
+    ms = ctypes.c_float()
+    out = []
+    for _ in range(reps):
+        sj.check(sj.lib().spj_d2d(b.ptr, a.ptr, nbytes, ctypes.byref(ms)))
+        out.append(ms.value)
+    a.free()
+    b.free()
+    print(json.dumps({"copy_bytes": nbytes, "ms": out, "GB/s_read_plus_write": [round(2 * nbytes / (m * 1e-3) / 1e9, 1)
+                                                                              for m in out]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +160,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--extra", action="store_true", help="also run the skewed 2^26 x 2^28 and sort-merge runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 1 GiB device-to-device copies (the plain copy kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and print their bandwidth")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -153,6 +172,9 @@ def main():
     sj.init(0)
     info = sj.device_info()
     log(f"[bench] device {info}")
+    if args.copy_calibration:
+        copy_calibration()
+        return
     nR, nS, zipf = 1 << 26, 1 << 27, 0.5
     keep = {}
     res = run_workload(nR, nS, zipf, args.steps, args.warmup, None, 1000, keep)

@@ -25,6 +25,7 @@ constexpr uint32_t kNoRid = 0xFFFFFFFFu;    // heavy key with no R tuple yet
 constexpr uint32_t kHeavySlots = 2048;      // heavy (key -> R rid) table, LDS resident
 constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
+constexpr uint32_t kTwoLevelBits = 8;       // from 256 partitions on, scatter in two levels
 
 // error bits in err[inst]
 constexpr uint32_t kErrDuplicateKey = 1u;   // a2a_srj_build.c:265-271 (VALIDATION)
@@ -44,6 +45,7 @@ struct InstDesc {
     uint64_t pstartS;        // offset of S partition starts (F+1 entries)
     uint32_t *heavy_keys;    // kHeavySlots keys (0 = empty)
     uint32_t *heavy_rids;    // kHeavySlots R rids (kNoRid = none)
+    const uint32_t *heavy_count;   // number of heavy keys found (device)
     uint32_t table_log2;     // LDS hash table slots (log2) for this instance
     uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop
 };

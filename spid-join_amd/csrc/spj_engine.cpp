@@ -81,8 +81,8 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     uint32_t b = nR > 16384 ? ceil_log2((uint64_t(nR) + 16383) / 16384) : 0;
     r.bits = std::min<uint32_t>(b, kMaxBits);
     r.ipt = 16;
-    // Tiles: at least one scatter round; at most 2^24 counters per relation.
     uint64_t n = std::max(nR, nS);
+    // Tiles: at least one scatter round; at most 2^24 counters per relation.
     uint64_t t = 8192;
     while ((n / t) * (uint64_t(1) << r.bits) > (uint64_t(1) << 24)) t <<= 1;
     r.tile = (uint32_t)t;
@@ -105,7 +105,7 @@ Context::~Context()
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_, &rp_,
-                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_};
+                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &scratch_};
     for (auto *b : bufs) b->release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -129,6 +129,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     max_bits_ = 0;
     max_table_ = 0;
     max_ipt_ = 8;
+    max_tile_ = 0;
+    two_level_ = false;
     any_heavy_ = false;
 
     // ---- plan: regions, tiles, scan segments
@@ -145,6 +147,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         max_bits_ = std::max(max_bits_, c.bits);
         max_table_ = std::max(max_table_, c.table_log2);
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
+        max_tile_ = std::max(max_tile_, c.tile);
+        two_level_ = two_level_ || (c.two_level && c.bits >= kTwoLevelBits);
         rp_off[i] = rp_total;
         sp_off[i] = sp_total;
         rp_total += s.nR;
@@ -223,6 +227,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         InstDesc &d = h_insts_[i];
         d.Rp = rp_.as<uint2>() + rp_off[i];
         d.Sp = sp_.as<uint2>() + sp_off[i];
+        d.heavy_count = ctrl_.as<uint32_t>() + ctrl_at(n, kCtrlHeavy, i);
         if (heavy_idx[i] >= 0) {
             d.heavy_keys = heavy_.as<uint32_t>() + (size_t)heavy_idx[i] * 2 * kHeavySlots;
             d.heavy_rids = d.heavy_keys + kHeavySlots;
@@ -266,8 +271,15 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
                         blk_b_.as<ScanBlock>(), (uint32_t)h_b_.size(), counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
-    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
-                           max_ipt_, st));
+    if (two_level_) {
+        const uint32_t grid = (uint32_t)std::max(1, device_info().cus);
+        SPJ_TRY(scratch_.ensure((size_t)grid * max_tile_ * 8));
+        SPJ_TRY(launch_scatter2(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), pstart_.as<uint32_t>(),
+                                scratch_.as<uint2>(), max_tile_, grid, ctrl + 3, out_hi, err, st));
+    } else {
+        SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
+                               max_ipt_, st));
+    }
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
     // ---- join

@@ -22,6 +22,7 @@ struct Resolved {
     uint32_t ipt = 16;        // scatter items per thread (round = 512*ipt)
     uint32_t tile = 65536;    // tuples per tile of the partition pass
     uint32_t smax = 65536;    // S tuples per work item
+    uint32_t two_level = 0;   // two-level scatter (scratch-staged) for >= 256 partitions
 };
 
 struct InstSpec {
@@ -82,7 +83,9 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, scratch_;
+    uint32_t max_tile_ = 0;
+    bool two_level_ = false;
     uint32_t max_holes_ = 0;
     size_t specs_n_ = 0;
     std::vector<uint32_t> h_ns_;

@@ -43,29 +43,23 @@ __device__ __forceinline__ uint32_t sfh(uint32_t key)
     return h;
 }
 
-// CRC-32 (reflected, polynomial 0xEDB88320) table built into LDS by the
-// workgroup itself: 256 threads, 8 shift/xor steps each.
-__device__ __forceinline__ void build_crc_lds(uint32_t *crc)
+// Bucket of a key in the join's LDS table: Fibonacci (multiply-shift)
+// hashing of the key.  Partitions are chosen by SuperFastHash bits, so the
+// two are independent.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_buckets)
 {
-    for (uint32_t n = threadIdx.x; n < 256; n += blockDim.x) {
-        uint32_t c = n;
-#pragma unroll
-        for (int k = 0; k < 8; k++) c = (c & 1u) ? (0xEDB88320u ^ (c >> 1)) : (c >> 1);
-        crc[n] = c;
-    }
+    return (key * 0x9E3779B1u) >> (32 - log2_buckets);
 }
 
-// join_hash: (0xd9d65adc & T[b0]) ^ T[b1] ^ T[b2] ^ T[b3] (hash.c:72-83).
-// Its low bits pick the LDS hash-table slot, as in a2a_srj_build.c:209-212.
-__device__ __forceinline__ uint32_t join_hash(const uint32_t *crc, uint32_t k)
+// Heavy table: kHeavySlots / 4 buckets of 4 slots; a key lives in one of
+// two buckets (2-choice), so a lookup is at most two 16-byte LDS reads and
+// never loops.
+constexpr uint32_t kHeavyBucketBits = 9;   // 512 buckets x 4 slots = kHeavySlots
+constexpr uint32_t kHeavyFilterLog2 = 16;  // 64K-bit (8 KB) membership bitmap
+__device__ __forceinline__ uint32_t heavy_b1(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - kHeavyBucketBits); }
+__device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
 {
-    return (0xd9d65adcu & crc[k & 0xFFu]) ^ crc[(k >> 8) & 0xFFu] ^ crc[(k >> 16) & 0xFFu] ^ crc[k >> 24];
-}
-
-// Heavy-table slot: multiplicative hash (the table is this build's own).
-__device__ __forceinline__ uint32_t heavy_slot(uint32_t key)
-{
-    return (key * 0x9E3779B1u) >> (32 - 11);   // kHeavySlots = 2048
+    return ((key ^ 0x5BD1E995u) * 0x85EBCA77u) >> (32 - kHeavyBucketBits);
 }
 
 // Sub-table of a partition that needs Q > 1 LDS tables: high SFH bits, which
@@ -128,25 +122,42 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
     return x * q + min(x, r) + j;
 }
 
-// Load the instance's heavy table into LDS.
-__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr)
+// Load the instance's heavy table into LDS and build its membership bitmap
+// (kHeavyFilterBits bits; a key whose bit is clear is not heavy).
+__device__ __forceinline__ uint32_t heavy_bit(uint32_t key) { return (key * 0xC2B2AE35u) >> (32 - kHeavyFilterLog2); }
+
+__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr, uint32_t *hf)
 {
+    for (uint32_t i = threadIdx.x; i < (1u << kHeavyFilterLog2) / 32; i += blockDim.x) hf[i] = 0;
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        hk[i] = I.heavy_keys[i];
+        const uint32_t k = I.heavy_keys[i];
+        hk[i] = k;
         hr[i] = I.heavy_rids[i];
+        if (k != kEmptyKey) {
+            const uint32_t b = heavy_bit(k);
+            atomicOr(&hf[b >> 5], 1u << (b & 31u));
+        }
     }
 }
 
-// Slot of key in the LDS heavy table, or -1.
-__device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key)
+// Slot of key (never 0) in the LDS heavy table, or -1.  hk is 16-byte aligned.
+__device__ __forceinline__ int heavy_find(const uint32_t *hk, const uint32_t *hf, uint32_t key)
 {
-    uint32_t s = heavy_slot(key);
-    for (uint32_t step = 0; step < kHeavySlots; step++) {
-        uint32_t k = hk[s];
-        if (k == key) return (int)s;
-        if (k == kEmptyKey) return -1;
-        s = (s + 1u) & (kHeavySlots - 1u);
-    }
+    const uint32_t f = heavy_bit(key);
+    if (!((hf[f >> 5] >> (f & 31u)) & 1u)) return -1;
+    const uint4 *hb = reinterpret_cast<const uint4 *>(hk);
+    const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
+    const uint4 a = hb[b1];
+    if (a.x == key) return (int)(4 * b1);
+    if (a.y == key) return (int)(4 * b1 + 1);
+    if (a.z == key) return (int)(4 * b1 + 2);
+    if (a.w == key) return (int)(4 * b1 + 3);
+    const uint4 c = hb[b2];
+    if (c.x == key) return (int)(4 * b2);
+    if (c.y == key) return (int)(4 * b2 + 1);
+    if (c.z == key) return (int)(4 * b2 + 2);
+    if (c.w == key) return (int)(4 * b2 + 3);
     return -1;
 }
 
@@ -172,12 +183,13 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
     const TileDesc td = tiles[t];
     const InstDesc &I = insts[td.inst];
     const uint32_t F = 1u << I.bits, mask = F - 1u;
-    uint32_t *hist = lds;                 // F
-    uint32_t *hk = lds + F;               // kHeavySlots
+    uint32_t *hk = lds;                   // kHeavySlots (16-byte aligned)
     uint32_t *hr = hk + kHeavySlots;      // kHeavySlots
-    const bool heavy = I.heavy != 0;
+    uint32_t *hf = hr + kHeavySlots;      // heavy bitmap
+    uint32_t *hist = hf + (1u << kHeavyFilterLog2) / 32;   // F
+    const bool heavy = I.heavy != 0 && *I.heavy_count != 0;
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
-    if (heavy) load_heavy(I, hk, hr);
+    if (heavy) load_heavy(I, hk, hr, hf);
     __syncthreads();
     const uint2 *in = td.rel ? I.S : I.R;
     const uint32_t mode = !heavy ? 0u : (td.rel == 0 ? 1u : 2u);
@@ -192,10 +204,10 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
 #pragma unroll
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
-            if (mode == 2 && heavy_find(hk, v[j].x) >= 0) continue;   // heavy S tuple: no partition
+            if (mode == 2 && heavy_find(hk, hf, v[j].x) >= 0) continue;   // heavy S tuple: no partition
             atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
             if (mode == 1) {
-                const int s = heavy_find(hk, v[j].x);
+                const int s = heavy_find(hk, hf, v[j].x);
                 if (s >= 0) I.heavy_rids[s] = v[j].y;   // R keys are unique: one writer per slot
             }
         }
@@ -299,22 +311,23 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
     const TileDesc td = tiles[t];
     const InstDesc &I = insts[td.inst];
     const uint32_t F = 1u << I.bits, mask = F - 1u;
-    const bool heavy = (I.heavy != 0) && (td.rel == 1);
+    const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
     // LDS layout
-    uint2 *stage = reinterpret_cast<uint2 *>(lds);                  // ROUND tuples
+    uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
+    uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
+    uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
+    uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
     uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
     uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // F + 2: counts -> starts
     uint32_t *off = lst + F + 2;                                    // F running write offsets
     uint32_t *bs = off + F;                                         // F: off[d] - start[d] this round
     uint32_t *wtmp = bs + F;                                        // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
-    uint32_t *hk = misc + 4;                                        // kHeavySlots
-    uint32_t *hr = hk + kHeavySlots;
 
     const uint32_t *cnt = counts + td.cnt_off;
     for (uint32_t i = threadIdx.x; i < F; i += NT) off[i] = cnt[i];
     for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-    if (heavy) load_heavy(I, hk, hr);
+    if (heavy) load_heavy(I, hk, hr, hf);
     const uint2 *in = td.rel ? I.S : I.R;
     uint2 *outp = td.rel ? I.Sp : I.Rp;
 
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
             dg[j] = DROP;
             if (li < rn && v[j].x != kEmptyKey) {
                 if (heavy) {
-                    const int s = heavy_find(hk, v[j].x);
+                    const int s = heavy_find(hk, hf, v[j].x);
                     if (s >= 0) {
                         // heavy: bucket F, unless the key has no R tuple
                         if (hr[s] != kNoRid) { dg[j] = F; v[j].x = hr[s]; }
@@ -361,7 +374,17 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
                     dg[j] = sfh(v[j].x) & mask;
                 }
             }
-            if (dg[j] != DROP) rk[j] = atomicAdd(&lst[dg[j]], 1u);
+            if (heavy) {
+                // heavy tuples all share bucket F: one LDS atomic per wave
+                const uint64_t hm = __ballot(dg[j] == F);
+                if (hm) {
+                    uint32_t hb = 0;
+                    if (lane_id() == 0) hb = atomicAdd(&lst[F], (uint32_t)__popcll(hm));
+                    hb = __shfl(hb, 0, kWave);
+                    if (dg[j] == F) rk[j] = hb + mbcnt(hm);
+                }
+            }
+            if (dg[j] != DROP && dg[j] != F) rk[j] = atomicAdd(&lst[dg[j]], 1u);
         }
         __syncthreads();
         const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
@@ -396,6 +419,222 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
         }
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter2: two-level radix scatter for large fan-out (bits >= 8).
+// With F = 4096 digits a round of 8K tuples leaves ~2 tuples per digit, so a
+// one-level scatter writes 16-byte runs and the memory system writes whole
+// lines for them.  Here a persistent workgroup takes a tile (up to 64K
+// tuples) and
+//   A. counting-sorts it by the top radix bits (coarse bucket, <= 64 of
+//      them) into its own scratch slot -- ~1 KB runs; the slot is reused for
+//      every tile, so it stays in L2 / the memory-side cache;
+//   B. scatters the coarse-sorted scratch by the full digit: a round of 8K
+//      consecutive scratch tuples spans few coarse buckets, so each digit
+//      gets a run of ~tile/F tuples (128 B at 64K / 4096).
+// Heavy S tuples are emitted as pairs in stage A, as in k_scatter.
+// ---------------------------------------------------------------------------
+template <int IPT>
+__global__ __launch_bounds__(512) void k_scatter2(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
+                                                  uint32_t ntiles, const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ pstart, uint2 *__restrict__ scratch,
+                                                  uint32_t scratch_per_wg, uint32_t *__restrict__ tile_ctr,
+                                                  uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t NT = 512, ROUND = NT * IPT;
+    constexpr uint32_t DROP = 0xFFFFFFFFu;
+    constexpr uint32_t CMAX = 64;
+    uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
+    uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
+    uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
+    uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
+    uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
+    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // 4096 + 2
+    uint32_t *off = lst + 4096 + 2;                                 // 4096 running write offsets
+    uint32_t *bs = off + 4096;                                      // 4096
+    uint32_t *cs = bs + 4096;                                       // CMAX + 1 coarse starts in scratch
+    uint32_t *wtmp = cs + CMAX + 1;                                 // NT/64 + 1
+    uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy base, [1] tile
+    uint2 *scr = scratch + (size_t)blockIdx.x * scratch_per_wg;
+
+    for (;;) {
+        if (threadIdx.x == 0) misc[1] = atomicAdd(tile_ctr, 1u);
+        __syncthreads();
+        const uint32_t t = misc[1];
+        if (t >= ntiles) break;
+        const TileDesc td = tiles[t];
+        const InstDesc &I = insts[td.inst];
+        const uint32_t bits = I.bits, F = 1u << bits, mask = F - 1u;
+        const uint32_t cbits = bits / 2, fbits = bits - cbits, C = 1u << cbits;
+        const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
+        const uint2 *in = td.rel ? I.S : I.R;
+        uint2 *outp = td.rel ? I.Sp : I.Rp;
+        const uint32_t rel_len = td.rel ? I.nS : I.nR;
+        const uint32_t *cnt = counts + td.cnt_off;
+        const uint32_t *pst = pstart + (td.rel ? I.pstartS : I.pstartR);
+        // fine offsets of this tile, and its fine counts (next tile's offset,
+        // or the partition end for the relation's last tile)
+        for (uint32_t d = threadIdx.x; d < F; d += NT) {
+            const uint32_t o = cnt[d];
+            const uint32_t nx = (td.end == rel_len) ? pst[d + 1] : cnt[F + d];
+            off[d] = o;
+            bs[d] = nx - o;   // count, reused as a temporary
+        }
+        for (uint32_t i = threadIdx.x; i < C + 2; i += NT) lst[i] = 0;
+        if (heavy) load_heavy(I, hk, hr, hf);
+        __syncthreads();
+        // coarse bucket sizes -> starts in scratch
+        for (uint32_t c = threadIdx.x; c < C; c += NT) {
+            uint32_t sum = 0;
+            for (uint32_t f = 0; f < (1u << fbits); f++) sum += bs[(c << fbits) + f];
+            cs[c] = sum;
+        }
+        __syncthreads();
+        const uint32_t ntile = block_exclusive_scan(cs, C, wtmp);   // cs[c] = start of coarse bucket c
+        if (threadIdx.x == 0) cs[C] = ntile;
+
+        // ---- stage A: input -> scratch, grouped by coarse bucket
+        uint2 nv[IPT];
+        {
+            const uint32_t rn = min(ROUND, td.end - td.begin);
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                if (li < rn) nv[j] = in[td.begin + li];
+            }
+        }
+        __syncthreads();
+        for (uint32_t rb = td.begin; rb < td.end; rb += ROUND) {
+            const uint32_t rn = min(ROUND, td.end - rb);
+            uint2 v[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) v[j] = nv[j];
+            if (rb + ROUND < td.end) {
+                const uint32_t nb = rb + ROUND, nn = min(ROUND, td.end - nb);
+#pragma unroll
+                for (int j = 0; j < IPT; j++) {
+                    const uint32_t li = j * NT + threadIdx.x;
+                    if (li < nn) nv[j] = in[nb + li];
+                }
+            }
+            uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                dg[j] = DROP;
+                if (li < rn && v[j].x != kEmptyKey) {
+                    if (heavy) {
+                        const int s = heavy_find(hk, hf, v[j].x);
+                        if (s >= 0) {
+                            if (hr[s] != kNoRid) { dg[j] = C; v[j].x = hr[s]; }
+                        } else {
+                            dg[j] = (sfh(v[j].x) & mask) >> fbits;
+                        }
+                    } else {
+                        dg[j] = (sfh(v[j].x) & mask) >> fbits;
+                    }
+                }
+                if (heavy) {
+                    const uint64_t hm = __ballot(dg[j] == C);
+                    if (hm) {
+                        uint32_t hb = 0;
+                        if (lane_id() == 0) hb = atomicAdd(&lst[C], (uint32_t)__popcll(hm));
+                        hb = __shfl(hb, 0, kWave);
+                        if (dg[j] == C) rk[j] = hb + mbcnt(hm);
+                    }
+                }
+                if (dg[j] != DROP && dg[j] != C) rk[j] = atomicAdd(&lst[dg[j]], 1u);
+            }
+            __syncthreads();
+            const uint32_t total = block_exclusive_scan(lst, C + 1, wtmp);
+            const uint32_t hstart = lst[C];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                if (dg[j] != DROP) {
+                    const uint32_t p = lst[dg[j]] + rk[j];
+                    stage[p] = v[j];
+                    sdig[p] = (uint16_t)dg[j];
+                }
+            }
+            for (uint32_t c = threadIdx.x; c < C; c += NT) {
+                const uint32_t s0 = lst[c], s1 = (c + 1 < C) ? lst[c + 1] : hstart;
+                bs[c] = cs[c] - s0;   // scratch position = bs[c] + staged index
+                cs[c] += s1 - s0;
+            }
+            if (heavy && threadIdx.x == 0) {
+                const uint32_t hn = total - hstart;
+                misc[0] = hn ? I.nS - atomicAdd(&out_hi[td.inst], hn) - hn : 0u;
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < hstart; i += NT) scr[bs[sdig[i]] + i] = stage[i];
+            if (heavy) {
+                const uint32_t hb = misc[0];
+                for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
+                    const uint32_t o = hb + (i - hstart);
+                    if (o < I.nS) I.out[o] = stage[i];
+                    else atomicOr(&err[td.inst], kErrOutOverflow);
+                }
+            }
+            for (uint32_t i = threadIdx.x; i < C + 2; i += NT) lst[i] = 0;
+            __syncthreads();
+        }
+        // scratch writes must be visible to the whole workgroup before stage B
+        __threadfence_block();
+        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
+        __syncthreads();
+
+        // ---- stage B: scratch -> partitions, by full digit
+        {
+            const uint32_t rn = min(ROUND, ntile);
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                if (li < rn) nv[j] = scr[li];
+            }
+        }
+        for (uint32_t rb = 0; rb < ntile; rb += ROUND) {
+            const uint32_t rn = min(ROUND, ntile - rb);
+            uint2 v[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) v[j] = nv[j];
+            if (rb + ROUND < ntile) {
+                const uint32_t nb = rb + ROUND, nn = min(ROUND, ntile - nb);
+#pragma unroll
+                for (int j = 0; j < IPT; j++) {
+                    const uint32_t li = j * NT + threadIdx.x;
+                    if (li < nn) nv[j] = scr[nb + li];
+                }
+            }
+            uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                dg[j] = (li < rn) ? (sfh(v[j].x) & mask) : DROP;
+                if (dg[j] != DROP) rk[j] = atomicAdd(&lst[dg[j]], 1u);
+            }
+            __syncthreads();
+            (void)block_exclusive_scan(lst, F, wtmp);
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                if (dg[j] != DROP) {
+                    const uint32_t p = lst[dg[j]] + rk[j];
+                    stage[p] = v[j];
+                    sdig[p] = (uint16_t)dg[j];
+                }
+            }
+            for (uint32_t d = threadIdx.x; d < F; d += NT) {
+                const uint32_t s0 = lst[d], s1 = (d + 1 < F) ? lst[d + 1] : rn;
+                bs[d] = off[d] - s0;
+                off[d] += s1 - s0;
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < rn; i += NT) outp[bs[sdig[i]] + i] = stage[i];
+            for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
+            __syncthreads();
+        }
     }
 }
 
@@ -455,24 +694,46 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
                                                        const uint32_t *__restrict__ cand_cnt, uint32_t min_count,
                                                        uint32_t *__restrict__ nheavy)
 {
-    __shared__ uint32_t n;
+    __shared__ uint32_t n, placed;
+    __shared__ uint32_t sel[kHeavyMaxKeys];
+    __shared__ __align__(16) uint32_t tab[kHeavySlots];
     const InstDesc &I = insts[inst];
     if (threadIdx.x == 0) n = 0;
+    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) tab[i] = kEmptyKey;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kCandSlots; i += blockDim.x) {
         if (cand_keys[i] == kEmptyKey || cand_cnt[i] < min_count) continue;
-        if (atomicAdd(&n, 1u) >= kHeavyMaxKeys) continue;
-        const uint32_t key = cand_keys[i];
-        uint32_t s = heavy_slot(key);
-        for (uint32_t step = 0; step < kHeavySlots; step++) {
-            uint32_t old = atomicCAS(&I.heavy_keys[s], kEmptyKey, key);
-            if (old == kEmptyKey) { I.heavy_rids[s] = kNoRid; break; }
-            if (old == key) break;
-            s = (s + 1u) & (kHeavySlots - 1u);
-        }
+        const uint32_t k = atomicAdd(&n, 1u);
+        if (k < kHeavyMaxKeys) sel[k] = cand_keys[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) *nheavy = min(n, kHeavyMaxKeys);
+    if (threadIdx.x == 0) {
+        // 2-choice placement into the emptier of the key's two 4-slot buckets;
+        // a key with both buckets full stays an ordinary key
+        uint32_t p = 0;
+        const uint32_t m = min(n, kHeavyMaxKeys);
+        for (uint32_t k = 0; k < m; k++) {
+            const uint32_t key = sel[k];
+            const uint32_t b[2] = {heavy_b1(key), heavy_b2(key)};
+            uint32_t fill[2];
+            for (int c = 0; c < 2; c++) {
+                fill[c] = 0;
+                for (uint32_t s = 0; s < 4; s++) fill[c] += tab[4 * b[c] + s] != kEmptyKey;
+            }
+            const int c = fill[1] < fill[0] ? 1 : 0;
+            if (fill[c] < 4) {
+                tab[4 * b[c] + fill[c]] = key;
+                p++;
+            }
+        }
+        placed = p;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
+        I.heavy_keys[i] = tab[i];
+        I.heavy_rids[i] = kNoRid;
+    }
+    if (threadIdx.x == 0) *nheavy = placed;
 }
 
 // ---------------------------------------------------------------------------
@@ -521,11 +782,11 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 // k_join: persistent workgroups pop items from the queue.  Per item the
 // algorithm is chosen by partition size and the instance's setting:
 //   hash (a2a_srj_build.c + a2a_srj_probe_inner_mt.c): R_p is counting-
-//     sorted into an LDS table by bucket = join_hash & 0xFFF -- the
-//     reference's 4096-slot table index -- so one bucket holds every key that
-//     would have probed from that slot; a probe scans one bucket (first equal
-//     key wins).  Load factor 1: up to kTableCap R tuples per workgroup.  A
-//     larger R_p is split into Q sub-tables by high hash bits;
+//     sorted into an LDS table of 4096 buckets (the reference's table has
+//     4096 slots) by a multiply-shift hash of the key; a probe reads one
+//     packed bucket word and scans the bucket (first equal key wins, as the
+//     reference's linear probe).  Load factor 1: up to kTableCap R tuples per
+//     workgroup.  A larger R_p is split into Q sub-tables by high hash bits;
 //   nested loop (nested_loop_join.c) for tiny R_p;
 //   sort-merge (mpsm_join_sort_probe.c): R_p and S chunks bitonic-sorted in
 //     LDS, then a per-thread merge.
@@ -537,9 +798,10 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 constexpr uint32_t kJoinNT = 1024;
 constexpr uint32_t kJoinIPT = 8;
 constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
-constexpr uint32_t kBuckets = 4096;         // join_hash & 0xFFF (a2a_srj_build.c:209-212)
+constexpr uint32_t kBuckets = 4096;         // at most 4096 buckets (the reference's 4096 slots)
 constexpr uint32_t kTableCap = 17408;       // 8 B per tuple + 16 KB of bucket ends
-constexpr uint32_t kSortChunk = 8192;       // sort-merge chunk (u64 words) per side
+constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
+constexpr uint32_t kSortS = 2048;           // sort-merge: S chunk sorted per merge
 constexpr uint32_t kNLChunk = 8192;
 
 struct JoinShared {
@@ -547,36 +809,22 @@ struct JoinShared {
     uint32_t fill;   // pairs written in the item's reservation
     uint32_t base;   // reservation start
     uint32_t res;    // reservation length
-    uint32_t ok;     // table built (no overflow)
 };
 
-// Append this thread's hits of one round (bit j of hit: pair rr[j], sr[j]).
-// Positions come from per-instruction ballots; one LDS atomic per wave.
-template <int IPT>
-__device__ __forceinline__ void wave_append(const uint32_t *rr, const uint32_t *sr, uint32_t hit, JoinShared *sh,
-                                            uint2 *out, uint32_t *err_i)
+// Append one pair per lane whose `hit` is set.  Positions come from a
+// ballot; one LDS atomic per wave instruction.
+__device__ __forceinline__ void wave_emit(bool hit, uint32_t r_rid, uint32_t s_rid, JoinShared *sh, uint2 *out,
+                                          uint32_t *err_i)
 {
-    uint64_t m[IPT];
-    uint32_t wtot = 0;
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        m[j] = __ballot((hit >> j) & 1u);
-        wtot += __popcll(m[j]);
-    }
-    if (wtot == 0) return;
+    const uint64_t m = __ballot(hit);
+    if (m == 0) return;
     uint32_t wb = 0;
-    if (lane_id() == 0) wb = atomicAdd(&sh->fill, wtot);
+    if (lane_id() == 0) wb = atomicAdd(&sh->fill, (uint32_t)__popcll(m));
     wb = __shfl(wb, 0, kWave);
-    const uint32_t lim = sh->res;
-    uint32_t o = wb;
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        if ((hit >> j) & 1u) {
-            const uint32_t pos = o + mbcnt(m[j]);
-            if (pos < lim) out[sh->base + pos] = make_uint2(rr[j], sr[j]);
-            else atomicOr(err_i, kErrOutOverflow);
-        }
-        o += __popcll(m[j]);
+    if (hit) {
+        const uint32_t pos = wb + mbcnt(m);
+        if (pos < sh->res) out[sh->base + pos] = make_uint2(r_rid, s_rid);
+        else atomicOr(err_i, kErrOutOverflow);
     }
 }
 
@@ -616,10 +864,8 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 {
     extern __shared__ uint32_t lds[];
     __shared__ JoinShared sh;
-    __shared__ uint32_t crc[256];
     __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
     constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
-    build_crc_lds(crc);
     const uint32_t total_items = ctrl[0];
     for (;;) {
         __syncthreads();
@@ -653,26 +899,32 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
                     load_round<IPT>(v, I.Sp, s0, w.se);
-                    uint32_t rr[IPT], sr[IPT], hit = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
-                        if (v[j].x == kEmptyKey) continue;
-                        for (uint32_t i = 0; i < cn; i++) {
-                            if (rk[i] == v[j].x) { rr[j] = rv[i]; sr[j] = v[j].y; hit |= 1u << j; break; }
+                        bool hit = false;
+                        uint32_t rr = 0;
+                        if (v[j].x != kEmptyKey) {
+                            for (uint32_t i = 0; i < cn; i++) {
+                                if (rk[i] == v[j].x) { rr = rv[i]; hit = true; break; }
+                            }
                         }
+                        wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
                     }
-                    wave_append<IPT>(rr, sr, hit, &sh, I.out, erri);
                 }
             }
         } else if (alg == 2) {
-            // ---------------- sort-merge: R chunk sorted once, each S chunk
-            // sorted and merged against it (an S tuple takes the rid of the
-            // equal R key, mpsm_join_sort_probe.c:214-236)
+            // ---------------- sort-merge (mpsm_join_sort_probe.c:152-236):
+            // the R partition is bitonic-sorted in LDS once; each S chunk is
+            // sorted and merged against it.  The merge of the sorted S chunk
+            // with the sorted R run is split among threads by merge path
+            // (a binary search on each thread's cross diagonal); a thread then
+            // merges its segment and emits one pair per S tuple whose key
+            // equals the current R key.
             uint64_t *ra = reinterpret_cast<uint64_t *>(lds);
-            uint64_t *sa = ra + kSortChunk;
-            uint32_t *cnts = reinterpret_cast<uint32_t *>(sa + kSortChunk);   // NT
-            for (uint32_t c0 = 0; c0 < nr; c0 += kSortChunk) {
-                const uint32_t cn = min(kSortChunk, nr - c0);
+            uint64_t *sa = ra + kSortR;
+            uint32_t *cnts = reinterpret_cast<uint32_t *>(sa + kSortS);   // NT
+            for (uint32_t c0 = 0; c0 < nr; c0 += kSortR) {
+                const uint32_t cn = min(kSortR, nr - c0);
                 uint32_t np = 1;
                 while (np < cn) np <<= 1;
                 __syncthreads();
@@ -682,46 +934,63 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 __syncthreads();
                 bitonic_sort_lds(ra, np);
-                for (uint32_t s0 = w.sb; s0 < w.se; s0 += kSortChunk) {
-                    const uint32_t sn = min(kSortChunk, w.se - s0);
+                for (uint32_t s0 = w.sb; s0 < w.se; s0 += kSortS) {
+                    const uint32_t sn = min(kSortS, w.se - s0);
                     uint32_t sp = 1;
                     while (sp < sn) sp <<= 1;
                     for (uint32_t i = threadIdx.x; i < sp; i += NT) {
-                        if (i < sn) { uint2 s = I.Sp[s0 + i]; sa[i] = ((uint64_t)s.x << 32) | s.y; }
+                        if (i < sn) { uint2 sv = I.Sp[s0 + i]; sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
                         else sa[i] = ~0ull;
                     }
                     __syncthreads();
                     bitonic_sort_lds(sa, sp);
-                    // merge: thread owns S[a, b); start from lower_bound in R
-                    const uint32_t per = (sn + NT - 1) / NT;
-                    const uint32_t a = min(sn, threadIdx.x * per), b = min(sn, a + per);
-                    uint32_t c = 0, r0 = 0;
-                    if (a < b) {
-                        const uint32_t k0 = (uint32_t)(sa[a] >> 32);
-                        uint32_t lo = 0, hi = cn;
-                        while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if ((uint32_t)(ra[mid] >> 32) < k0) lo = mid + 1; else hi = mid; }
-                        r0 = lo;
-                        uint32_t r = r0;
-                        for (uint32_t i = a; i < b; i++) {
-                            const uint32_t k = (uint32_t)(sa[i] >> 32);
-                            while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
-                            if (k != kEmptyKey && r < cn && (uint32_t)(ra[r] >> 32) == k) c++;
+                    // merge path over A = S[0, sn), B = R[0, cn) ordered by key
+                    const uint32_t L = sn + cn, per = (L + NT - 1) / NT;
+                    const uint32_t d0 = min(L, threadIdx.x * per), d1 = min(L, d0 + per);
+                    uint32_t ai = 0, bi = 0, c = 0;
+                    {
+                        // a = number of S elements among the first d0 merged ones:
+                        // the first m whose S key exceeds R[d0-1-m] (S first on ties)
+                        uint32_t lo = d0 > cn ? d0 - cn : 0u, hi = min(d0, sn);
+                        while (lo < hi) {
+                            const uint32_t m = (lo + hi) >> 1;
+                            const uint32_t ka = (uint32_t)(sa[m] >> 32), kb = (uint32_t)(ra[d0 - 1 - m] >> 32);
+                            if (ka <= kb) lo = m + 1; else hi = m;
+                        }
+                        ai = lo;
+                        bi = d0 - lo;
+                    }
+                    const uint32_t a_start = ai, b_start = bi;
+                    for (uint32_t d = d0; d < d1; d++) {
+                        // take from S when its key <= the R key: an S tuple is
+                        // consumed while the equal R key is still current
+                        const bool takeA = ai < sn && (bi >= cn || (uint32_t)(sa[ai] >> 32) <= (uint32_t)(ra[bi] >> 32));
+                        if (takeA) {
+                            const uint32_t k = (uint32_t)(sa[ai] >> 32);
+                            if (k != kEmptyKey && bi < cn && (uint32_t)(ra[bi] >> 32) == k) c++;
+                            ai++;
+                        } else {
+                            bi++;
                         }
                     }
                     cnts[threadIdx.x] = c;
                     __syncthreads();
                     const uint32_t tot = block_exclusive_scan(cnts, NT, wtmp);
                     uint32_t o = sh.fill + cnts[threadIdx.x];
-                    if (a < b) {
-                        uint32_t r = r0;
-                        for (uint32_t i = a; i < b; i++) {
-                            const uint32_t k = (uint32_t)(sa[i] >> 32);
-                            while (r < cn && (uint32_t)(ra[r] >> 32) < k) r++;
-                            if (k != kEmptyKey && r < cn && (uint32_t)(ra[r] >> 32) == k) {
-                                if (o < sh.res) I.out[sh.base + o] = make_uint2((uint32_t)ra[r], (uint32_t)sa[i]);
+                    ai = a_start;
+                    bi = b_start;
+                    for (uint32_t d = d0; d < d1; d++) {
+                        const bool takeA = ai < sn && (bi >= cn || (uint32_t)(sa[ai] >> 32) <= (uint32_t)(ra[bi] >> 32));
+                        if (takeA) {
+                            const uint32_t k = (uint32_t)(sa[ai] >> 32);
+                            if (k != kEmptyKey && bi < cn && (uint32_t)(ra[bi] >> 32) == k) {
+                                if (o < sh.res) I.out[sh.base + o] = make_uint2((uint32_t)ra[bi], (uint32_t)sa[ai]);
                                 else atomicOr(erri, kErrOutOverflow);
                                 o++;
                             }
+                            ai++;
+                        } else {
+                            bi++;
                         }
                     }
                     __syncthreads();
@@ -731,59 +1000,100 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             }
         } else {
             // ---------------- hash join
-            uint32_t *tk = lds, *tv = lds + kTableCap, *be = lds + 2 * kTableCap;
+            // LDS: (key, rid) pairs counting-sorted by bucket, then one packed
+            // word per bucket: start << 16 | size (both < 2^16 = kTableCap bound)
+            uint2 *tkv = reinterpret_cast<uint2 *>(lds);
+            uint32_t *be = lds + 2 * kTableCap;
             const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
-            const uint32_t nb = 1u << min(I.table_log2, 12u), bm = nb - 1u;   // buckets
+            const uint32_t lb = min(I.table_log2, 12u), nb = 1u << lb;   // buckets
             for (uint32_t q = 0; q < Q; q++) {
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < nb; i += NT) be[i] = 0;
                 __syncthreads();
-                // count pass: tuples per bucket
-                for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                    uint2 r[4];
+                if (Q == 1) {
+                    // R_p fits one table: hold it in registers (<= 17 tuples per
+                    // thread), count buckets, scan, then place -- one read of R_p
+                    constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
+                    uint2 r[RPT];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const uint32_t i = i0 + u * NT + threadIdx.x;
+                    for (uint32_t u = 0; u < RPT; u++) {
+                        const uint32_t i = rb + u * NT + threadIdx.x;
                         r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (r[u].x == kEmptyKey) continue;
-                        if (Q > 1 && sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
-                        atomicAdd(&be[join_hash(crc, r[u].x) & bm], 1u);
-                    }
-                }
-                __syncthreads();
-                const uint32_t nq = block_exclusive_scan(be, nb, wtmp);   // be[b] = start of bucket b
-                if (nq > kTableCap) {
-                    if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
-                    continue;   // uniform: nq is the same in every thread
-                }
-                // fill pass: be[b] advances to the end of bucket b
-                for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                    uint2 r[4];
+                    for (uint32_t u = 0; u < RPT; u++)
+                        if (r[u].x != kEmptyKey) atomicAdd(&be[bucket_of(r[u].x, lb)], 1u);
+                    __syncthreads();
+                    (void)block_exclusive_scan(be, nb, wtmp);
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const uint32_t i = i0 + u * NT + threadIdx.x;
-                        r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                    for (uint32_t u = 0; u < RPT; u++) {
                         if (r[u].x == kEmptyKey) continue;
-                        if (Q > 1 && sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
-                        const uint32_t p = atomicAdd(&be[join_hash(crc, r[u].x) & bm], 1u);
-                        tk[p] = r[u].x;
-                        tv[p] = r[u].y;
+                        tkv[atomicAdd(&be[bucket_of(r[u].x, lb)], 1u)] = r[u];
                     }
+                    __syncthreads();
+                } else {
+                    // count pass: tuples per bucket
+                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                        uint2 r[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const uint32_t i = i0 + u * NT + threadIdx.x;
+                            r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            if (r[u].x == kEmptyKey || sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
+                            atomicAdd(&be[bucket_of(r[u].x, lb)], 1u);
+                        }
+                    }
+                    __syncthreads();
+                    const uint32_t nq = block_exclusive_scan(be, nb, wtmp);   // be[b] = start of bucket b
+                    if (nq > kTableCap) {
+                        if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
+                        continue;   // uniform: nq is the same in every thread
+                    }
+                    // fill pass: be[b] advances to the end of bucket b
+                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                        uint2 r[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const uint32_t i = i0 + u * NT + threadIdx.x;
+                            r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            if (r[u].x == kEmptyKey || sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
+                            tkv[atomicAdd(&be[bucket_of(r[u].x, lb)], 1u)] = r[u];
+                        }
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
+                // be[b] holds the end of bucket b: pack start << 16 | size
+                {
+                    uint32_t pk[(4096 + NT - 1) / NT];
+#pragma unroll
+                    for (uint32_t u = 0; u < (4096 + NT - 1) / NT; u++) {
+                        const uint32_t b = u * NT + threadIdx.x;
+                        if (b < nb) {
+                            const uint32_t st = b ? be[b - 1] : 0u, en = be[b];
+                            pk[u] = (st << 16) | (en - st);
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (uint32_t u = 0; u < (4096 + NT - 1) / NT; u++) {
+                        const uint32_t b = u * NT + threadIdx.x;
+                        if (b < nb) be[b] = pk[u];
+                    }
+                    __syncthreads();
+                }
                 // duplicate R keys (a2a_srj_build.c:265-271): pairwise within each bucket
                 for (uint32_t b = threadIdx.x; b < nb; b += NT) {
-                    const uint32_t s = b ? be[b - 1] : 0u, e = be[b];
-                    if (e - s > 64) continue;
-                    for (uint32_t i = s; i < e; i++)
-                        for (uint32_t k = i + 1; k < e; k++)
-                            if (tk[i] == tk[k]) atomicOr(erri, kErrDuplicateKey);
+                    const uint32_t st = be[b] >> 16, en = st + (be[b] & 0xFFFFu);
+                    if (en - st > 64) continue;
+                    for (uint32_t i = st; i < en; i++)
+                        for (uint32_t k = i + 1; k < en; k++)
+                            if (tkv[i].x == tkv[k].x) atomicOr(erri, kErrDuplicateKey);
                 }
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
@@ -793,19 +1103,21 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
                     if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, s0 + ROUND, w.se);
-                    uint32_t rr[IPT], sr[IPT], hit = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
                         const uint32_t k = v[j].x;
-                        if (k == kEmptyKey) continue;
-                        if (Q > 1 && sub_of(sfh(k), I.bits, Q) != q) continue;
-                        const uint32_t b = join_hash(crc, k) & bm;
-                        const uint32_t e = be[b];
-                        for (uint32_t i = b ? be[b - 1] : 0u; i < e; i++) {
-                            if (tk[i] == k) { rr[j] = tv[i]; sr[j] = v[j].y; hit |= 1u << j; break; }
+                        bool hit = false;
+                        uint32_t rr = 0;
+                        if (k != kEmptyKey && (Q == 1 || sub_of(sfh(k), I.bits, Q) == q)) {
+                            const uint32_t d = be[bucket_of(k, lb)];
+                            const uint32_t en = (d >> 16) + (d & 0xFFFFu);
+                            for (uint32_t i = d >> 16; i < en; i++) {
+                                const uint2 e = tkv[i];
+                                if (e.x == k) { rr = e.y; hit = true; break; }
+                            }
                         }
+                        wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
                     }
-                    wave_append<IPT>(rr, sr, hit, &sh, I.out, erri);
                 }
             }
         }
@@ -840,18 +1152,29 @@ static hipError_t set_lds(const void *fn, size_t bytes)
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-size_t lds_hist(uint32_t max_bits) { return ((size_t(1) << max_bits) + 2 * kHeavySlots) * 4; }
+size_t lds_hist(uint32_t max_bits)
+{
+    return ((size_t(1) << max_bits) + 2 * kHeavySlots + (1u << kHeavyFilterLog2) / 32) * 4;
+}
+
+size_t lds_scatter2(int ipt)
+{
+    const size_t R = 512u * (size_t)ipt;
+    return R * 8 + R * 2 + (4096 + 2) * 4 + 2 * 4096 * 4 + (64 + 1) * 4 + (512 / kWave + 1) * 4 + 16 +
+           2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8;
+}
 
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = 512u * (size_t)ipt;
-    return R * 8 + R * 2 + (F + 2) * 4 + 2 * F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4;
+    return R * 8 + R * 2 + (F + 2) * 4 + 2 * F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4 +
+           (1u << kHeavyFilterLog2) / 8;
 }
 
 size_t lds_join()
 {
     size_t hash = (2 * (size_t)kTableCap + kBuckets) * 4;
-    size_t smj = 2 * kSortChunk * 8 + kJoinNT * 4;
+    size_t smj = ((size_t)kSortR + kSortS) * 8 + kJoinNT * 4;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
     return m > nl ? m : nl;
@@ -909,6 +1232,19 @@ hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<8>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_hi, err);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
+                           const uint32_t *pstart, uint2 *scratch, uint32_t scratch_per_wg, uint32_t grid,
+                           uint32_t *tile_ctr, uint32_t *out_hi, uint32_t *err, hipStream_t st)
+{
+    if (!ntiles) return hipSuccess;
+    size_t l = lds_scatter2(16);
+    hipError_t e = set_lds((const void *)k_scatter2<16>, l);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scatter2<16>, dim3(grid), dim3(512), l, st, insts, tiles, ntiles, counts, pstart, scratch,
+                       scratch_per_wg, tile_ctr, out_hi, err);
     return hipGetLastError();
 }
 

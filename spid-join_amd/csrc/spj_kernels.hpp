@@ -22,6 +22,12 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
                        uint32_t max_bits, hipStream_t st);
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
                           uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st);
+// Two-level scatter (radix bits 8..12), persistent over `grid` workgroups
+// with one scratch slot of scratch_per_wg tuples each.
+hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
+                           const uint32_t *pstart, uint2 *scratch, uint32_t scratch_per_wg, uint32_t grid,
+                           uint32_t *tile_ctr, uint32_t *out_hi, uint32_t *err, hipStream_t st);
+size_t lds_scatter2(int ipt);
 hipError_t launch_heavy_detect(const InstDesc *insts, uint32_t inst, uint32_t nS, uint32_t sample_blocks,
                                uint32_t min_count, uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *nheavy,
                                hipStream_t st);
