@@ -153,6 +153,100 @@ def copy_calibration(nbytes=1 << 30, reps=3):
                                                                               for m in out]}), flush=True)
 
 
+def batched(args):
+    """Batched mixed joins: instances with |R| log-uniform in [2^10, 2^max_log2r],
+    R:S from {1,2,4,8}, Zipf U[0, 1.25].  One batched launch sequence
+    (persistent join kernel over one work queue) against one join per call,
+    with identical per-instance results (count and pair-multiset checksum)."""
+    import ctypes
+    rng = np.random.default_rng(4096)
+    n = args.instances
+    nRs = np.floor(2.0 ** rng.uniform(10, args.max_log2r, n)).astype(np.int64)
+    ratios = rng.choice([1, 2, 4, 8], n)
+    zipfs = rng.uniform(0, 1.25, n)
+    nSs = nRs * ratios
+    totR, totS = int(nRs.sum()), int(nSs.sum())
+    t0 = time.time()
+    dR = sj.DeviceArray(totR * 8)
+    dS = sj.DeviceArray(totS * 8)
+    dO = sj.DeviceArray(totS * 8)
+    offR = np.concatenate([[0], np.cumsum(nRs)[:-1]])
+    offS = np.concatenate([[0], np.cumsum(nSs)[:-1]])
+    for i in range(n):
+        R = sj.gen_r(int(nRs[i]), 10_000 + i)
+        S = sj.gen_s(int(nSs[i]), R, float(zipfs[i]), 20_000 + i)
+        sj.check(sj.lib().spj_h2d(dR.ptr + int(offR[i]) * 8, R.ctypes.data, R.nbytes))
+        sj.check(sj.lib().spj_h2d(dS.ptr + int(offS[i]) * 8, S.ctypes.data, S.nbytes))
+    log(f"[bench] batched: {n} instances, |R| total {totR}, |S| total {totS}; data ready in {time.time() - t0:.1f}s")
+    P = ctypes.c_void_p
+    rp = (P * n)(*[dR.ptr + int(o) * 8 for o in offR])
+    sp = (P * n)(*[dS.ptr + int(o) * 8 for o in offS])
+    op = (P * n)(*[dO.ptr + int(o) * 8 for o in offS])
+    nr = (ctypes.c_uint64 * n)(*[int(x) for x in nRs])
+    ns = (ctypes.c_uint64 * n)(*[int(x) for x in nSs])
+    cnt = (ctypes.c_uint64 * n)()
+    L = sj.lib()
+
+    def run_batch():
+        sj.check(L.spj_join_batch_device(n, ctypes.cast(rp, P), ctypes.cast(nr, P), ctypes.cast(sp, P),
+                                         ctypes.cast(ns, P), ctypes.cast(op, P), ctypes.cast(cnt, P), None, None))
+
+    def checksums():
+        return [sj.checksum_device(int(op[i]), int(cnt[i])) for i in range(n)]
+
+    for _ in range(args.warmup):
+        run_batch()
+    sj.check(L.spj_dev_sync())
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        run_batch()
+    sj.check(L.spj_dev_sync())
+    t_batch = (time.perf_counter() - t1) / args.steps
+    counts_b = [int(cnt[i]) for i in range(n)]
+    sums_b = checksums()
+    # one join per call
+    lat = []
+    one = sj.lib().spj_join_device
+    c1 = ctypes.c_uint64()
+    for i in range(min(n, 16)):   # warm-up
+        sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
+    sj.check(L.spj_dev_sync())
+    counts_1 = []
+    t2 = time.perf_counter()
+    for i in range(n):
+        ta = time.perf_counter()
+        sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
+        lat.append(time.perf_counter() - ta)
+        counts_1.append(int(c1.value))
+    t_single = time.perf_counter() - t2
+    for i in range(n):
+        cnt[i] = counts_1[i]
+    sums_1 = checksums()
+    identical = counts_b == counts_1 and sums_b == sums_1 and counts_b == [int(x) for x in nSs]
+    lat_ms = np.array(lat) * 1e3
+    dR.free()
+    dS.free()
+    dO.free()
+    tuples = totR + totS
+    out = {
+        "metric": "batched joined tuples/s (sum |R|+|S| over instances)",
+        "value": round(tuples / t_batch, 1), "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "uint32",
+        "data": "synthetic (seeded: |R| log-uniform, R:S in {1,2,4,8}, Zipf U[0,1.25])",
+        "config": {"workload": f"batched mixed joins, {n} instances, |R| in [2^10, 2^{args.max_log2r}]",
+                   "instances": n, "sum_R": totR, "sum_S": totS},
+        "one_join_per_call": {"tuples_per_s": round(tuples / t_single, 1), "total_ms": round(t_single * 1e3, 2),
+                              "latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 4),
+                              "latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 4)},
+        "speedup_batched_over_one_per_call": round(t_single / t_batch, 2),
+        "identical_per_instance_results": bool(identical),
+    }
+    print(json.dumps(out), flush=True)
+    if not identical:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +257,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device-to-device copies (the plain copy kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print their bandwidth")
+    ap.add_argument("--batched", action="store_true", help="batched mixed-joins workload instead of the default")
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--max-log2r", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -174,6 +271,9 @@ def main():
     log(f"[bench] device {info}")
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.batched:
+        batched(args)
         return
     nR, nS, zipf = 1 << 26, 1 << 27, 0.5
     keep = {}

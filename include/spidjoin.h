@@ -85,6 +85,9 @@ int spj_dev_memset(void *ptr, int value, uint64_t bytes);
  * (used to calibrate achievable HBM bandwidth). */
 int spj_d2d(void *dst, const void *src, uint64_t bytes, float *ms);
 int spj_dev_sync(void);
+/* Order-independent checksum of device-resident pairs: sum and xor of a
+ * 64-bit mix of (r_rid << 32 | s_rid) -- compares result multisets in place. */
+int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum, uint64_t *xr);
 
 /* One join on device-resident relations: the whole plan of
  * execution.hpp:15-157 (partition R, partition S, join) on one stream.

@@ -13,6 +13,7 @@
 #include "spidjoin.h"
 #include "spj_capi_internal.hpp"
 #include "spj_engine.hpp"
+#include "spj_kernels.hpp"
 #include "spj_tuner.hpp"
 
 namespace spj {
@@ -194,6 +195,20 @@ int spj_d2d(void *dst, const void *src, uint64_t bytes, float *ms)
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
     return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipMemcpy D2D");
+}
+
+int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum, uint64_t *xr)
+{
+    if ((!pairs && n) || !sum || !xr) return fail(SPJ_EINVAL, "null argument");
+    unsigned long long *acc = nullptr, h[2] = {0, 0};
+    hipError_t e = hipMalloc(&acc, 16);
+    if (e == hipSuccess) e = launch_pairs_checksum(reinterpret_cast<const uint2 *>(pairs), n, acc, 0);
+    if (e == hipSuccess) e = hipMemcpy(h, acc, 16, hipMemcpyDeviceToHost);
+    if (acc) (void)hipFree(acc);
+    if (e != hipSuccess) return hip_fail(e, "checksum");
+    *sum = h[0];
+    *xr = h[1];
+    return SPJ_OK;
 }
 
 int spj_dev_sync(void)

@@ -83,6 +83,19 @@ struct ScanBlock {
 
 constexpr uint32_t kScanTileChunk = 32;   // tiles per column-scan chunk
 
+// Heavy-key detection of one instance: its sample blocks are
+// first_block .. first_block + nblocks - 1 of the sampling grid, every
+// stride-th 8K-tuple block of S; its candidate map lives at cand_off.
+struct HeavyJob {
+    uint32_t inst;
+    uint32_t stride;
+    uint32_t first_block;
+    uint32_t nblocks;
+    uint32_t min_count;
+    uint32_t pad;
+    uint64_t cand_off;
+};
+
 // One unit of join work: partition p of instance inst, S range [sb, se)
 // (absolute positions in the instance's partitioned S region).
 struct WorkItem {

@@ -105,7 +105,7 @@ Context::~Context()
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_, &rp_,
-                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &scratch_};
+                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &scratch_, &jobs_};
     for (auto *b : bufs) b->release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -221,7 +221,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
     if (nheavy_inst) {
         SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
-        SPJ_TRY(cand_.ensure((size_t)2 * cand_slots() * 4));
+        SPJ_TRY(cand_.ensure((size_t)nheavy_inst * 2 * cand_slots() * 4));
     }
     for (size_t i = 0; i < n; i++) {
         InstDesc &d = h_insts_[i];
@@ -251,15 +251,27 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
     SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
 
-    // ---- heavy keys
+    // ---- heavy keys: one sampling launch and one selection launch for all
     if (nheavy_inst) {
-        SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
+        h_jobs_.clear();
+        uint32_t blocks = 0;
         for (size_t i = 0; i < n; i++) {
             if (heavy_idx[i] < 0) continue;
-            SPJ_TRY(launch_fill(cand_.as<uint32_t>(), 2ull * cand_slots(), 0u, st));
-            SPJ_TRY(launch_heavy_detect(d_insts, (uint32_t)i, specs[i].nS, 1024, specs[i].cfg.heavy_min,
-                                        cand_.as<uint32_t>(), cand_.as<uint32_t>() + cand_slots(), nheavy + i, st));
+            HeavyJob j{};
+            j.inst = (uint32_t)i;
+            j.first_block = blocks;
+            j.nblocks = heavy_sample_blocks(specs[i].nS, 1024, &j.stride);
+            j.min_count = specs[i].cfg.heavy_min;
+            j.cand_off = (uint64_t)heavy_idx[i] * 2 * cand_slots();
+            blocks += j.nblocks;
+            h_jobs_.push_back(j);
         }
+        SPJ_TRY(jobs_.ensure(h_jobs_.size() * sizeof(HeavyJob)));
+        SPJ_TRY(hipMemcpyAsync(jobs_.p, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob), hipMemcpyHostToDevice, st));
+        SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
+        SPJ_TRY(launch_fill(cand_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * cand_slots(), 0u, st));
+        SPJ_TRY(launch_heavy_detect(d_insts, jobs_.as<HeavyJob>(), (uint32_t)h_jobs_.size(), blocks,
+                                    cand_.as<uint32_t>(), nheavy, st));
     }
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 

@@ -83,7 +83,8 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, scratch_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, scratch_, jobs_;
+    std::vector<HeavyJob> h_jobs_;
     uint32_t max_tile_ = 0;
     bool two_level_ = false;
     uint32_t max_holes_ = 0;

@@ -654,17 +654,24 @@ constexpr uint32_t kSampleSlots = 16384;
 constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
 
-__global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict__ insts, uint32_t inst,
-                                                      uint32_t stride_blocks, uint32_t *__restrict__ cand_keys,
-                                                      uint32_t *__restrict__ cand_cnt)
+__global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
+                                                      uint32_t njobs, uint32_t *__restrict__ cand_base)
 {
     extern __shared__ uint32_t lds[];
     uint32_t *sk = lds;
     uint32_t *sc = lds + kSampleSlots;
-    const InstDesc &I = insts[inst];
+    // this block's job: last job whose first_block <= blockIdx.x
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].first_block <= blockIdx.x) lo = m; else hi = m;
+    }
+    const HeavyJob job = jobs[lo];
+    const InstDesc &I = insts[job.inst];
+    uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
     for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
     __syncthreads();
-    const uint64_t b = (uint64_t)blockIdx.x * stride_blocks * kSampleBlock;
+    const uint64_t b = (uint64_t)(blockIdx.x - job.first_block) * job.stride * kSampleBlock;
     const uint32_t e = (uint32_t)min<uint64_t>(I.nS, b + kSampleBlock);
     for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint32_t key = I.S[i].x;
@@ -689,15 +696,17 @@ __global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict
     }
 }
 
-__global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restrict__ insts, uint32_t inst,
-                                                       const uint32_t *__restrict__ cand_keys,
-                                                       const uint32_t *__restrict__ cand_cnt, uint32_t min_count,
+__global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ cand_base,
                                                        uint32_t *__restrict__ nheavy)
 {
     __shared__ uint32_t n, placed;
     __shared__ uint32_t sel[kHeavyMaxKeys];
     __shared__ __align__(16) uint32_t tab[kHeavySlots];
-    const InstDesc &I = insts[inst];
+    const HeavyJob job = jobs[blockIdx.x];
+    const InstDesc &I = insts[job.inst];
+    const uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
+    const uint32_t min_count = job.min_count;
     if (threadIdx.x == 0) n = 0;
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) tab[i] = kEmptyKey;
     __syncthreads();
@@ -733,7 +742,7 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
         I.heavy_keys[i] = tab[i];
         I.heavy_rids[i] = kNoRid;
     }
-    if (threadIdx.x == 0) *nheavy = placed;
+    if (threadIdx.x == 0) nheavy[job.inst] = placed;
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,6 +1153,39 @@ __global__ void k_gather_segments(const uint2 *__restrict__ src, uint2 *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Order-independent checksum of a pair array (sum and xor of a 64-bit mix of
+// r_rid << 32 | s_rid): lets a caller compare two result multisets without
+// copying or sorting them.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_pairs_checksum(const uint2 *__restrict__ p, uint64_t n,
+                                                        unsigned long long *__restrict__ acc)
+{
+    uint64_t s = 0, x = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint2 v = p[i];
+        const uint64_t h = mix64(((uint64_t)v.x << 32) | v.y);
+        s += h;
+        x ^= h;
+    }
+    for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, kWave);
+        x ^= __shfl_xor(x, off, kWave);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&acc[0], (unsigned long long)s);
+        atomicXor(&acc[1], (unsigned long long)x);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static hipError_t set_lds(const void *fn, size_t bytes)
@@ -1248,20 +1290,26 @@ hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_heavy_detect(const InstDesc *insts, uint32_t inst, uint32_t nS, uint32_t sample_blocks,
-                               uint32_t min_count, uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *nheavy,
-                               hipStream_t st)
+uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 {
-    uint32_t total_blocks = (nS + kSampleBlock - 1) / kSampleBlock;
-    if (!total_blocks) return hipSuccess;
-    uint32_t stride = total_blocks / sample_blocks;
-    if (stride < 1) stride = 1;
-    uint32_t grid = (total_blocks + stride - 1) / stride;
+    const uint32_t total = (nS + kSampleBlock - 1) / kSampleBlock;
+    // sample at most 1/16 of the blocks (and at most max_blocks of them)
+    const uint32_t want = min(max_blocks, (total + 15) / 16);
+    uint32_t st = want ? total / want : 1;
+    if (st < 1) st = 1;
+    if (stride) *stride = st;
+    return total ? (total + st - 1) / st : 0;
+}
+
+hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, uint32_t total_blocks,
+                               uint32_t *cand_base, uint32_t *nheavy, hipStream_t st)
+{
+    if (!njobs || !total_blocks) return hipSuccess;
     size_t l = 2 * kSampleSlots * 4;
     hipError_t e = set_lds((const void *)k_heavy_sample, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_heavy_sample, dim3(grid), dim3(512), l, st, insts, inst, stride, cand_keys, cand_cnt);
-    hipLaunchKernelGGL(k_heavy_select, dim3(1), dim3(1024), 0, st, insts, inst, cand_keys, cand_cnt, min_count, nheavy);
+    hipLaunchKernelGGL(k_heavy_sample, dim3(total_blocks), dim3(512), l, st, insts, jobs, njobs, cand_base);
+    hipLaunchKernelGGL(k_heavy_select, dim3(njobs), dim3(1024), 0, st, insts, jobs, cand_base, nheavy);
     return hipGetLastError();
 }
 
@@ -1284,6 +1332,16 @@ hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const Work
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, pstart, items, ctrl, out_lo, out_fill, err,
                        holes, max_holes);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st)
+{
+    hipError_t e = hipMemsetAsync(acc, 0, 16, st);
+    if (e != hipSuccess || !n) return e;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_pairs_checksum, dim3((uint32_t)blocks), dim3(256), 0, st, p, n, acc);
     return hipGetLastError();
 }
 

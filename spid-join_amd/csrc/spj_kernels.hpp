@@ -28,14 +28,20 @@ hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_
                            const uint32_t *pstart, uint2 *scratch, uint32_t scratch_per_wg, uint32_t grid,
                            uint32_t *tile_ctr, uint32_t *out_hi, uint32_t *err, hipStream_t st);
 size_t lds_scatter2(int ipt);
-hipError_t launch_heavy_detect(const InstDesc *insts, uint32_t inst, uint32_t nS, uint32_t sample_blocks,
-                               uint32_t min_count, uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *nheavy,
-                               hipStream_t st);
+// Sample blocks for heavy detection of an S of nS tuples (at most
+// max_blocks, evenly strided); *stride receives the stride in blocks.
+uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride);
+// Heavy-key detection for every job at once: one sampling launch over all
+// jobs' sample blocks, one selection workgroup per job.  Each job owns
+// 2 * cand_slots() words of zeroed candidate map at cand_base + cand_off.
+hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, uint32_t total_blocks,
+                               uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
                              WorkItem *items, uint32_t *nitems, hipStream_t st);
 hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st);
+hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st);
 hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *segs, uint32_t nsegs, hipStream_t st);
 
 }  // namespace spj

@@ -23,7 +23,7 @@ ERRORS = {0: "OK", -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EDUPKEY", -5: "E
 # Every function the C header declares (checked by the CPU tests).
 EXPORTS = [
     "spj_config_default", "spj_last_error", "spj_init", "spj_device_info", "spj_dev_alloc", "spj_dev_free",
-    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_dev_sync", "spj_join_device", "spj_join_host",
+    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_pairs_checksum_device", "spj_dev_sync", "spj_join_device", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_free_result", "spj_destroy",
     "spj_tuner_set", "spj_tuner_get", "spj_tuner_clear", "spj_tuner_entries", "spj_tuner_save",
@@ -85,6 +85,7 @@ def lib():
         "spj_dev_memset": (ctypes.c_int, [P, ctypes.c_int, U64]),
         "spj_dev_sync": (ctypes.c_int, []),
         "spj_d2d": (ctypes.c_int, [P, P, U64, ctypes.POINTER(ctypes.c_float)]),
+        "spj_pairs_checksum_device": (ctypes.c_int, [P, U64, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
         "spj_join_device": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config),
                                            ctypes.POINTER(PhaseTimes)]),
         "spj_join_host": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config)]),
@@ -186,6 +187,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def checksum_device(ptr, n):
+    """(sum, xor) of the pairs at device address ptr (see spj_pairs_checksum_device)."""
+    s, x = U64(), U64()
+    check(lib().spj_pairs_checksum_device(ptr, n, ctypes.byref(s), ctypes.byref(x)))
+    return s.value, x.value
 
 
 def init(device=0):

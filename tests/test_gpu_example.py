@@ -1,0 +1,23 @@
+"""GPU: the reference driver's JoinInstance flow (test/join/main.cc) through
+the C++ facade, built by __graft_entry__.build() as spid-join_amd/join_example."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "spid-join_amd", "join_example")
+
+
+@pytest.mark.parametrize("args", [["-s", "1048576", "-t", "2"], ["-s", "262144", "-t", "8", "-z", "150", "-B", "8"],
+                                  ["-s", "131072", "-t", "4", "-z", "50", "-a", "2"]])
+def test_reference_driver_flow(args):
+    if not os.path.exists(EXE):
+        pytest.fail("join_example not built (run __graft_entry__.build())")
+    print("case", args, flush=True)
+    r = subprocess.run([EXE] + args, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "bad 0" in r.stdout
+    assert "JOIN" in r.stdout   # RecordLogs timeline

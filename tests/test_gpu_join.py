@@ -138,3 +138,27 @@ def test_batch_mixed_two_level(gpu, orc):
     outs, _ = sj.join_batch(Rs, Ss)
     for R, S, p in zip(Rs, Ss, outs):
         check_equal(sj, orc, R, S, p)
+
+
+@pytest.mark.parametrize("nR,nS,zipf", [(1 << 26, 1 << 27, 0.5), (1 << 26, 1 << 28, 1.5)])
+def test_full_size_checksum(gpu, orc, nR, nS, zipf):
+    """Full-size parity through a size-independent property: the pair count
+    and the order-independent checksum of the GPU output (computed on the
+    device) equal the oracle's closed-form checksum."""
+    sj = gpu
+    print(f"case full nR={nR} nS={nS} zipf={zipf}", flush=True)
+    R = sj.gen_r(nR, 1000)
+    S = sj.gen_s(nS, R, zipf, 1001)
+    want = orc.join_checksum(R, S)
+    dR = sj.DeviceArray.from_numpy(R)
+    dS = sj.DeviceArray.from_numpy(S)
+    del S
+    dO = sj.DeviceArray(nS * 8)
+    try:
+        n, _ = sj.join_device(dR, nR, dS, nS, dO)
+        got = (n,) + sj.checksum_device(dO.ptr, n)
+    finally:
+        dR.free()
+        dS.free()
+        dO.free()
+    assert got == want
