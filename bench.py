@@ -247,6 +247,79 @@ def batched(args):
         sys.exit(1)
 
 
+def serving(args):
+    """Serving loop: 16 resident tables (|R| log-uniform [2^16, 2^24]), probe
+    requests with |S| log-uniform [2^12, 2^22], Zipf U[0, 1.5], random table,
+    arriving as a Poisson process at each swept rate.  Reports p50 / p99
+    latency (submission to result on the host) and probe tuples/s."""
+    import threading
+    rng = np.random.default_rng(16)
+    t0 = time.time()
+    ntab = 16
+    sizes = np.floor(2.0 ** rng.uniform(16, 24, ntab)).astype(np.int64)
+    tables = [sj.gen_r(int(n), 30_000 + i) for i, n in enumerate(sizes)]
+    pool = []
+    for i in range(args.requests):
+        t = int(rng.integers(0, ntab))
+        n = int(2 ** rng.uniform(12, 22))
+        pool.append((t, sj.gen_s(n, tables[t], float(rng.uniform(0, 1.5)), 40_000 + i)))
+    sv = sj.Server(args.budget_gb << 30, slots=args.slots)
+    tids = [sv.add_table(R) for R in tables]
+    log(f"[bench] serving: {ntab} tables (sum |R| {int(sizes.sum())}), {len(pool)} requests "
+        f"(sum |S| {sum(len(s) for _, s in pool)}); ready in {time.time() - t0:.1f}s")
+    # warm-up
+    for t, S in pool[:8]:
+        sv.wait(sv.submit(tids[t], S))
+    results = []
+    for rate in args.rates:
+        gaps = rng.exponential(1.0 / rate, len(pool))
+        ids = [None] * len(pool)
+
+        def client():
+            nxt = time.perf_counter()
+            for k, (t, S) in enumerate(pool):
+                nxt += gaps[k]
+                d = nxt - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = sv.submit(tids[t], S)
+
+        th = threading.Thread(target=client)
+        ts = time.perf_counter()
+        th.start()
+        lats, probe = [], 0
+        for k in range(len(pool)):
+            while ids[k] is None:
+                time.sleep(1e-4)
+            out, lat = sv.wait(ids[k])
+            if len(out) != len(pool[k][1]):
+                raise RuntimeError("serving result has a wrong count")
+            lats.append(lat)
+            probe += len(pool[k][1])
+        th.join()
+        wall = time.perf_counter() - ts
+        st = sv.stats()
+        lats = np.array(lats)
+        r = {"rate_req_per_s": rate, "p50_ms": round(float(np.percentile(lats, 50)), 3),
+             "p99_ms": round(float(np.percentile(lats, 99)), 3), "probe_tuples_per_s": round(probe / wall, 1),
+             "wall_s": round(wall, 3)}
+        log(f"[bench]   rate {rate}/s: {r}")
+        results.append(r)
+    st = sv.stats()
+    sv.close()
+    best = max(results, key=lambda r: r["probe_tuples_per_s"])
+    out = {
+        "metric": "serving probe tuples/s (p50/p99 latency per rate)", "value": best["probe_tuples_per_s"],
+        "unit": "tuples/s", "n_gpus": 1, "steps": len(pool), "warmup": 8, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "uint32",
+        "data": "synthetic (seeded tables and Poisson request stream)",
+        "config": {"workload": "serving loop: 16 resident tables, Poisson probe requests",
+                   "tables": ntab, "requests": len(pool), "budget_gb": args.budget_gb, "slots": args.slots},
+        "sweep": results, "batches": st["batches"], "batched_requests": st["requests"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +333,11 @@ def main():
     ap.add_argument("--batched", action="store_true", help="batched mixed-joins workload instead of the default")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--max-log2r", type=float, default=20.0)
+    ap.add_argument("--serving", action="store_true", help="serving-loop workload instead of the default")
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rates", type=float, nargs="+", default=[50, 200, 800, 3200])
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -274,6 +352,9 @@ def main():
         return
     if args.batched:
         batched(args)
+        return
+    if args.serving:
+        serving(args)
         return
     nR, nS, zipf = 1 << 26, 1 << 27, 0.5
     keep = {}

@@ -144,6 +144,22 @@ int spj_last_phase_times(spj_instance_t *inst, spj_phase_times_t *times);
 void spj_free_result(spj_pair_t *pairs);
 void spj_destroy(spj_instance_t *inst);
 
+/* ---- serving: resident tables, budgeted admission, continuous batching ----
+ * No counterpart in the reference.  Tables are radix-partitioned once and
+ * stay in HBM; submitted probe batches (S) are admitted in arrival order
+ * while their HBM footprint fits the budget and packed into batches that
+ * run on `slots` concurrent streams. */
+typedef struct spj_server spj_server_t;
+spj_server_t *spj_server_create(uint64_t hbm_budget_bytes, int slots);
+int spj_server_add_table(spj_server_t *sv, const spj_tuple_t *R, uint64_t nR, int *table_id);
+/* S stays borrowed until spj_server_wait returns for this request. */
+int spj_server_submit(spj_server_t *sv, int table_id, const spj_tuple_t *S, uint64_t nS, uint64_t *request_id);
+/* Blocks until the request is done; pairs are freed with spj_free_result;
+ * latency_ms = completion minus submission. */
+int spj_server_wait(spj_server_t *sv, uint64_t request_id, spj_pair_t **pairs, uint64_t *count, double *latency_ms);
+int spj_server_stats(spj_server_t *sv, uint64_t *batches, uint64_t *requests);
+void spj_server_destroy(spj_server_t *sv);
+
 /* ---- autotuner: per size / skew bucket parameter table ---- */
 /* Set / get the parameters of the bucket (ceil log2 nR, ceil log2 nS/nR,
  * skew class of zipf) -- used by every later call whose config fields are -1. */

@@ -149,9 +149,10 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
         max_tile_ = std::max(max_tile_, c.tile);
         two_level_ = two_level_ || (c.two_level && c.bits >= kTwoLevelBits);
+        const bool pre = s.Rp_pre != nullptr;
         rp_off[i] = rp_total;
         sp_off[i] = sp_total;
-        rp_total += s.nR;
+        if (!pre) rp_total += s.nR;
         sp_total += s.nS;
         InstDesc &d = h_insts_[i];
         d.R = s.R;
@@ -160,7 +161,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         d.nR = s.nR;
         d.nS = s.nS;
         d.bits = c.bits;
-        d.heavy = (c.heavy && s.nS > 0) ? 1u : 0u;
+        d.heavy = (c.heavy && s.nS > 0 && !pre) ? 1u : 0u;
         d.table_log2 = c.table_log2;
         d.alg = c.alg;
         d.pstartR = pst_total;
@@ -171,7 +172,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
             any_heavy_ = true;
         }
         items_bound += F + (uint64_t)s.nS / smax + 1;
-        for (uint32_t rel = 0; rel < 2; rel++) {
+        for (uint32_t rel = pre ? 1u : 0u; rel < 2; rel++) {
             const uint32_t len = rel ? s.nS : s.nR;
             const uint32_t T = (len + c.tile - 1) / c.tile;
             ScanSeg sg{};
@@ -225,7 +226,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     }
     for (size_t i = 0; i < n; i++) {
         InstDesc &d = h_insts_[i];
-        d.Rp = rp_.as<uint2>() + rp_off[i];
+        d.Rp = specs[i].Rp_pre ? const_cast<uint2 *>(specs[i].Rp_pre) : rp_.as<uint2>() + rp_off[i];
         d.Sp = sp_.as<uint2>() + sp_off[i];
         d.heavy_count = ctrl_.as<uint32_t>() + ctrl_at(n, kCtrlHeavy, i);
         if (heavy_idx[i] >= 0) {
@@ -242,6 +243,10 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     if (!h_ad_.empty())
         SPJ_TRY(hipMemcpyAsync(blk_ad_.p, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
     SPJ_TRY(hipMemcpyAsync(blk_b_.p, h_b_.data(), h_b_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
+    for (size_t i = 0; i < n; i++)   // resident tables: their partition starts
+        if (specs[i].Rp_pre)
+            SPJ_TRY(hipMemcpyAsync(pstart_.as<uint32_t>() + h_insts_[i].pstartR, specs[i].pst_pre,
+                                   ((size_t(1) << specs[i].cfg.bits) + 1) * 4, hipMemcpyDeviceToDevice, st));
 
     const InstDesc *d_insts = insts_.as<InstDesc>();
     uint32_t *ctrl = ctrl_.as<uint32_t>();
@@ -390,6 +395,17 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         }
     }
     return hipSuccess;
+}
+
+hipError_t Context::export_partitioned_r(size_t i, uint2 *rp_dst, uint32_t *pstart_dst)
+{
+    if (i >= h_insts_.size()) return hipErrorInvalidValue;
+    const InstDesc &d = h_insts_[i];
+    const size_t F = size_t(1) << d.bits;
+    if (d.nR) SPJ_TRY(hipMemcpyAsync(rp_dst, d.Rp, (size_t)d.nR * 8, hipMemcpyDeviceToDevice, stream_));
+    SPJ_TRY(hipMemcpyAsync(pstart_dst, pstart_.as<uint32_t>() + d.pstartR, (F + 1) * 4, hipMemcpyDeviceToDevice,
+                           stream_));
+    return hipStreamSynchronize(stream_);
 }
 
 hipError_t Context::run(const std::vector<InstSpec> &specs, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs,

@@ -32,6 +32,11 @@ struct InstSpec {
     uint32_t nS = 0;
     uint2 *out = nullptr;     // capacity >= nS pairs
     Resolved cfg;
+    // Optional: R already radix-partitioned with cfg.bits (a resident table):
+    // Rp_pre holds the partitioned tuples and pst_pre (device) the F + 1
+    // partition starts.  Only S is partitioned then; heavy handling is off.
+    const uint2 *Rp_pre = nullptr;
+    const uint32_t *pst_pre = nullptr;
 };
 
 // Milliseconds per phase (HIP events on the context's stream).
@@ -78,6 +83,10 @@ public:
     // arrays returned by out_counts()/errs() when the stream reaches them.
     hipError_t enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed);
     hipError_t collect(size_t ninst, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed);
+
+    // After run()/collect() of a batch, copy instance i's partitioned R and
+    // its F + 1 partition starts to caller-owned device buffers.
+    hipError_t export_partitioned_r(size_t inst, uint2 *rp_dst, uint32_t *pstart_dst);
 
 private:
     hipStream_t stream_ = nullptr;

@@ -27,7 +27,8 @@ EXPORTS = [
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_free_result", "spj_destroy",
     "spj_tuner_set", "spj_tuner_get", "spj_tuner_clear", "spj_tuner_entries", "spj_tuner_save",
-    "spj_tuner_load", "spj_autotune", "spj_gen_r", "spj_gen_s",
+    "spj_tuner_load", "spj_autotune", "spj_gen_r", "spj_gen_s", "spj_server_create", "spj_server_add_table",
+    "spj_server_submit", "spj_server_wait", "spj_server_stats", "spj_server_destroy",
 ]
 
 
@@ -110,6 +111,13 @@ def lib():
         "spj_tuner_load": (ctypes.c_int, [ctypes.c_char_p]),
         "spj_autotune": (ctypes.c_int, [P, U64, P, U64, P, ctypes.c_double, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_float)]),
+        "spj_server_create": (P, [U64, ctypes.c_int]),
+        "spj_server_add_table": (ctypes.c_int, [P, P, U64, ctypes.POINTER(ctypes.c_int)]),
+        "spj_server_submit": (ctypes.c_int, [P, ctypes.c_int, P, U64, ctypes.POINTER(U64)]),
+        "spj_server_wait": (ctypes.c_int, [P, U64, ctypes.POINTER(P), ctypes.POINTER(U64),
+                                           ctypes.POINTER(ctypes.c_double)]),
+        "spj_server_stats": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
+        "spj_server_destroy": (None, [P]),
         "spj_gen_r": (ctypes.c_int, [P, U64, U64]),
         "spj_gen_s": (ctypes.c_int, [P, U64, P, U64, ctypes.c_double, U64, ctypes.c_int]),
     }
@@ -305,6 +313,58 @@ class JoinInstance:
     def close(self):
         if self.h:
             lib().spj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Serving loop: resident tables, HBM-budget admission, continuous batching."""
+
+    def __init__(self, hbm_budget_bytes, slots=2):
+        self.h = lib().spj_server_create(int(hbm_budget_bytes), int(slots))
+        if not self.h:
+            raise SpjError(-1, lib().spj_last_error().decode())
+        self._keep = {}
+
+    def add_table(self, R):
+        R = np.ascontiguousarray(R, dtype=TUPLE)
+        tid = ctypes.c_int()
+        check(lib().spj_server_add_table(self.h, R.ctypes.data if len(R) else None, len(R), ctypes.byref(tid)))
+        return tid.value
+
+    def submit(self, table, S):
+        S = np.ascontiguousarray(S, dtype=TUPLE)
+        rid = U64()
+        check(lib().spj_server_submit(self.h, int(table), S.ctypes.data if len(S) else None, len(S),
+                                      ctypes.byref(rid)))
+        self._keep[rid.value] = S
+        return rid.value
+
+    def wait(self, rid):
+        p = P()
+        n = U64()
+        lat = ctypes.c_double()
+        try:
+            check(lib().spj_server_wait(self.h, rid, ctypes.byref(p), ctypes.byref(n), ctypes.byref(lat)))
+            out = np.empty(n.value, dtype=PAIR)
+            if n.value:
+                ctypes.memmove(out.ctypes.data, p.value, n.value * 8)
+            return out, lat.value
+        finally:
+            if p.value:
+                lib().spj_free_result(p)
+            self._keep.pop(rid, None)
+
+    def stats(self):
+        b, r = U64(), U64()
+        check(lib().spj_server_stats(self.h, ctypes.byref(b), ctypes.byref(r)))
+        return {"batches": b.value, "requests": r.value}
+
+    def close(self):
+        if self.h:
+            lib().spj_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
