@@ -96,9 +96,23 @@ def roofline(res):
     ach = byts / (ms * 1e-3)
     b_alg = 24 * (nR + nS) + 8 * res["pairs"]
     return name, {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": round(ach / HBM_PEAK, 4), "traffic": None}, {
+                  "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(name)}, {
         k: {"ms": round(v[0], 4), "alg_bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)}
         for k, v in kern.items()}, b_alg
+
+
+def measured_traffic(kernel):
+    """Bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    plain-copy calibration) from the committed rocprofv3 counter summary of
+    this same workload (profiles/c2_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_traffic.json")
+    key = {"k_scatter": "k_scatter<16>"}.get(kernel, kernel)
+    try:
+        with open(path) as f:
+            t = json.load(f)["per_dispatch"][key]
+        return int(t["fetch_bytes"] + t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(zipf, seconds=10.0):
