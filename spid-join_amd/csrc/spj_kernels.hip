@@ -808,7 +808,8 @@ constexpr uint32_t kJoinNT = 1024;
 constexpr uint32_t kJoinIPT = 8;
 constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
 constexpr uint32_t kBuckets = 4096;         // at most 4096 buckets (the reference's 4096 slots)
-constexpr uint32_t kTableCap = 17408;       // 8 B per tuple + 16 KB of bucket ends
+constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket words
+constexpr uint32_t kScan = 8;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
 constexpr uint32_t kSortS = 2048;           // sort-merge: S chunk sorted per merge
 constexpr uint32_t kNLChunk = 8192;
@@ -1011,14 +1012,15 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             // ---------------- hash join
             // LDS: (key, rid) pairs counting-sorted by bucket, then one packed
             // word per bucket: start << 16 | size (both < 2^16 = kTableCap bound)
-            uint2 *tkv = reinterpret_cast<uint2 *>(lds);
-            uint32_t *be = lds + 2 * kTableCap;
+            uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
+            uint32_t *be = lds + 2 * (kTableCap + kScan);
             const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
             const uint32_t lb = min(I.table_log2, 12u), nb = 1u << lb;   // buckets
             for (uint32_t q = 0; q < Q; q++) {
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < nb; i += NT) be[i] = 0;
                 __syncthreads();
+                uint32_t ntab = 0;   // entries in this table
                 if (Q == 1) {
                     // R_p fits one table: hold it in registers (<= 17 tuples per
                     // thread), count buckets, scan, then place -- one read of R_p
@@ -1033,7 +1035,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     for (uint32_t u = 0; u < RPT; u++)
                         if (r[u].x != kEmptyKey) atomicAdd(&be[bucket_of(r[u].x, lb)], 1u);
                     __syncthreads();
-                    (void)block_exclusive_scan(be, nb, wtmp);
+                    ntab = block_exclusive_scan(be, nb, wtmp);
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         if (r[u].x == kEmptyKey) continue;
@@ -1057,6 +1059,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     }
                     __syncthreads();
                     const uint32_t nq = block_exclusive_scan(be, nb, wtmp);   // be[b] = start of bucket b
+                    ntab = nq;
                     if (nq > kTableCap) {
                         if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
                         continue;   // uniform: nq is the same in every thread
@@ -1096,13 +1099,18 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     }
                     __syncthreads();
                 }
-                // duplicate R keys (a2a_srj_build.c:265-271): pairwise within each bucket
-                for (uint32_t b = threadIdx.x; b < nb; b += NT) {
-                    const uint32_t st = be[b] >> 16, en = st + (be[b] & 0xFFFFu);
-                    if (en - st > 64) continue;
-                    for (uint32_t i = st; i < en; i++)
-                        for (uint32_t k = i + 1; k < en; k++)
-                            if (tkv[i].x == tkv[k].x) atomicOr(erri, kErrDuplicateKey);
+                // duplicate R keys (a2a_srj_build.c:265-271): each entry against
+                // the later entries of its bucket
+                {
+                    bool dup = false;
+                    for (uint32_t i = threadIdx.x; i < ntab; i += NT) {
+                        const uint32_t k = tkv[i].x;
+                        if (k == kEmptyKey) continue;
+                        const uint32_t d = be[bucket_of(k, lb)];
+                        const uint32_t en = (d >> 16) + (d & 0xFFFFu);
+                        for (uint32_t j = i + 1; j < en; j++) dup |= (tkv[j].x == k);
+                    }
+                    if (dup) atomicOr(erri, kErrDuplicateKey);
                 }
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
@@ -1119,10 +1127,18 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         uint32_t rr = 0;
                         if (k != kEmptyKey && (Q == 1 || sub_of(sfh(k), I.bits, Q) == q)) {
                             const uint32_t d = be[bucket_of(k, lb)];
-                            const uint32_t en = (d >> 16) + (d & 0xFFFFu);
-                            for (uint32_t i = d >> 16; i < en; i++) {
+                            const uint32_t st = d >> 16, n = d & 0xFFFFu;
+                            // first kScan entries without a data-dependent loop (the
+                            // table has kScan padding slots); a longer bucket goes on
+                            // in the tail loop
+#pragma unroll
+                            for (uint32_t u = 0; u < kScan; u++) {
+                                const uint2 e = tkv[st + u];
+                                if (u < n && e.x == k && !hit) { rr = e.y; hit = true; }
+                            }
+                            for (uint32_t i = st + kScan; i < st + n && !hit; i++) {
                                 const uint2 e = tkv[i];
-                                if (e.x == k) { rr = e.y; hit = true; break; }
+                                if (e.x == k) { rr = e.y; hit = true; }
                             }
                         }
                         wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
@@ -1215,7 +1231,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 
 size_t lds_join()
 {
-    size_t hash = (2 * (size_t)kTableCap + kBuckets) * 4;
+    size_t hash = (2 * ((size_t)kTableCap + kScan) + kBuckets) * 4;
     size_t smj = ((size_t)kSortR + kSortS) * 8 + kJoinNT * 4;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
