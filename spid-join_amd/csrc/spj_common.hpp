@@ -25,7 +25,8 @@ constexpr uint32_t kNoRid = 0xFFFFFFFFu;    // heavy key with no R tuple yet
 constexpr uint32_t kHeavySlots = 2048;      // heavy (key -> R rid) table, LDS resident
 constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
-constexpr uint32_t kTwoLevelBits = 8;       // from 256 partitions on, scatter in two levels
+constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
+constexpr uint32_t kScatterGroups = 8;      // scatter groups (one per XCD) per relation
 
 // error bits in err[inst]
 constexpr uint32_t kErrDuplicateKey = 1u;   // a2a_srj_build.c:265-271 (VALIDATION)
@@ -50,8 +51,9 @@ struct InstDesc {
     uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop
 };
 
-// One tile of one relation of one instance.  Its digit counts live at
-// counts[cnt_off .. cnt_off + F).
+// One tile of one relation of one instance.  cnt_off is the offset of the
+// F digit counters of the tile's scatter group: histogram counts before the
+// scan, write cursors after it.
 struct TileDesc {
     uint32_t inst;
     uint32_t rel;            // 0 = R, 1 = S
@@ -59,10 +61,9 @@ struct TileDesc {
     uint64_t cnt_off;
 };
 
-// One (instance, relation) whose tile counts are scanned together.  Its tiles
-// are t_first .. t_first + t_count - 1, with counts at
-// cnt_off0 + (t - t_first) * F; chunk partials live at part_off; partition
-// starts at pstart_off (F + 1 entries).
+// One (instance, relation) whose group counts are scanned together: t_count
+// groups (kScatterGroups), group g's counters at cnt_off0 + g * F; chunk
+// partials live at part_off; partition starts at pstart_off (F + 1 entries).
 struct ScanSeg {
     uint32_t inst;
     uint32_t rel;
@@ -73,7 +74,7 @@ struct ScanSeg {
     uint64_t pstart_off;
 };
 
-// One block of a column-scan phase: segment, 256-digit chunk, tile chunk.
+// One block of a column-scan phase: segment, 256-digit chunk, group chunk.
 struct ScanBlock {
     uint32_t seg;
     uint32_t dchunk;

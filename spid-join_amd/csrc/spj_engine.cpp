@@ -72,6 +72,16 @@ static uint32_t ceil_log2(uint64_t x)
     return r;
 }
 
+// Tiles of at least one scatter round, and about 2048 of them per relation
+// at most: enough for every scatter workgroup to take several, few enough
+// that the histogram's counter atomics stay cheap.
+uint32_t default_tile(uint64_t n)
+{
+    uint64_t t = 8192;
+    while (n / t > 2048) t <<= 1;
+    return (uint32_t)t;
+}
+
 Resolved default_params(uint32_t nR, uint32_t nS)
 {
     Resolved r;
@@ -80,12 +90,8 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     // room for the binomial spread of partition sizes.
     uint32_t b = nR > 16384 ? ceil_log2((uint64_t(nR) + 16383) / 16384) : 0;
     r.bits = std::min<uint32_t>(b, kMaxBits);
-    r.ipt = 16;
-    uint64_t n = std::max(nR, nS);
-    // Tiles: at least one scatter round; at most 2^24 counters per relation.
-    uint64_t t = 8192;
-    while ((n / t) * (uint64_t(1) << r.bits) > (uint64_t(1) << 24)) t <<= 1;
-    r.tile = (uint32_t)t;
+    r.ipt = 8;
+    r.tile = default_tile(std::max(nR, nS));
     r.smax = 65536;
     r.heavy = nS >= (1u << 20) ? 1u : 0u;
     r.heavy_min = 16;
@@ -105,7 +111,7 @@ Context::~Context()
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_, &rp_,
-                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &scratch_, &jobs_};
+                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &jobs_};
     for (auto *b : bufs) b->release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -128,9 +134,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     h_b_.clear();
     max_bits_ = 0;
     max_table_ = 0;
-    max_ipt_ = 8;
-    max_tile_ = 0;
-    two_level_ = false;
+    max_ipt_ = 4;
     any_heavy_ = false;
 
     // ---- plan: regions, tiles, scan segments
@@ -147,8 +151,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         max_bits_ = std::max(max_bits_, c.bits);
         max_table_ = std::max(max_table_, c.table_log2);
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
-        max_tile_ = std::max(max_tile_, c.tile);
-        two_level_ = two_level_ || (c.two_level && c.bits >= kTwoLevelBits);
         const bool pre = s.Rp_pre != nullptr;
         rp_off[i] = rp_total;
         sp_off[i] = sp_total;
@@ -179,7 +181,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
             sg.inst = (uint32_t)i;
             sg.rel = rel;
             sg.bits = c.bits;
-            sg.t_count = T;
+            sg.t_count = kScatterGroups;
             sg.cnt_off0 = cnt_total;
             sg.part_off = part_total;
             sg.pstart_off = rel ? d.pstartS : d.pstartR;
@@ -191,19 +193,29 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
                 td.rel = rel;
                 td.begin = t * c.tile;
                 td.end = std::min<uint64_t>(len, (uint64_t)(t + 1) * c.tile);
-                td.cnt_off = cnt_total + (uint64_t)t * F;
+                td.cnt_off = cnt_total;   // + group * F below
                 h_tiles_.push_back(td);
             }
-            const uint32_t dch = (F + 255) / 256, tch = (T + kScanTileChunk - 1) / kScanTileChunk;
+            const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
             for (uint32_t dc = 0; dc < dch; dc++) {
                 h_b_.push_back(ScanBlock{seg_id, dc, 0, 0});
                 for (uint32_t tc = 0; tc < tch; tc++) h_ad_.push_back(ScanBlock{seg_id, dc, tc, 0});
             }
-            cnt_total += (uint64_t)T * F;
+            cnt_total += (uint64_t)kScatterGroups * F;
             part_total += (uint64_t)tch * F;
         }
     }
     if (h_tiles_.size() > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    // scatter groups: the tiles of XCD x's range (as the scatter kernel walks
+    // them) share group x's counters
+    {
+        const uint64_t T = h_tiles_.size(), q = T / kScatterGroups, r = T % kScatterGroups;
+        for (uint64_t t = 0; t < T; t++) {
+            const uint64_t g = (t < r * (q + 1)) ? t / (q + 1) : r + (t - r * (q + 1)) / q;
+            TileDesc &td = h_tiles_[t];
+            td.cnt_off += g * (uint64_t(1) << h_insts_[td.inst].bits);
+        }
+    }
 
     // ---- buffers
     SPJ_TRY(insts_.ensure(n * sizeof(InstDesc)));
@@ -282,21 +294,16 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
 
     // ---- partition: histogram, scan, scatter
     const uint32_t ntiles = (uint32_t)h_tiles_.size();
+    SPJ_TRY(launch_fill(counts_.as<uint32_t>(), cnt_total, 0u, st));
     SPJ_TRY(launch_hist(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
     SPJ_TRY(launch_scan(segs_.as<ScanSeg>(), (uint32_t)h_segs_.size(), blk_ad_.as<ScanBlock>(), (uint32_t)h_ad_.size(),
                         blk_b_.as<ScanBlock>(), (uint32_t)h_b_.size(), counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
-    if (two_level_) {
-        const uint32_t grid = (uint32_t)std::max(1, device_info().cus);
-        SPJ_TRY(scratch_.ensure((size_t)grid * max_tile_ * 8));
-        SPJ_TRY(launch_scatter2(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), pstart_.as<uint32_t>(),
-                                scratch_.as<uint2>(), max_tile_, grid, ctrl + 3, out_hi, err, st));
-    } else {
-        SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
-                               max_ipt_, st));
-    }
+    // one persistent 1024-thread workgroup per CU (LDS-bound)
+    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
+                           max_ipt_, (uint32_t)std::max(1, device_info().cus), st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
     // ---- join

@@ -19,10 +19,9 @@ struct Resolved {
     uint32_t alg = 0;         // 0 hash (+NL for tiny), 2 sort-merge, 3 nested loop
     uint32_t heavy = 0;       // heavy-key detection and direct emission
     uint32_t heavy_min = 16;  // summed sample count for a key to be heavy
-    uint32_t ipt = 16;        // scatter items per thread (round = 512*ipt)
+    uint32_t ipt = 8;         // scatter tuples per thread (round = 1024 * ipt)
     uint32_t tile = 65536;    // tuples per tile of the partition pass
     uint32_t smax = 65536;    // S tuples per work item
-    uint32_t two_level = 0;   // two-level scatter (scratch-staged) for >= 256 partitions
 };
 
 struct InstSpec {
@@ -92,10 +91,8 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, scratch_, jobs_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, jobs_;
     std::vector<HeavyJob> h_jobs_;
-    uint32_t max_tile_ = 0;
-    bool two_level_ = false;
     uint32_t max_holes_ = 0;
     size_t specs_n_ = 0;
     std::vector<uint32_t> h_ns_;
@@ -106,11 +103,13 @@ private:
     std::vector<ScanBlock> h_ad_, h_b_;
     std::vector<uint32_t> h_ctrl_;
     uint32_t max_bits_ = 0, max_table_ = 0;
-    int max_ipt_ = 16;
+    int max_ipt_ = 8;
     bool any_heavy_ = false;
 };
 
 // Default parameters for an instance of the given sizes (no tuner entry).
 Resolved default_params(uint32_t nR, uint32_t nS);
+// Default partition-pass tile size for a relation pair whose larger side has n tuples.
+uint32_t default_tile(uint64_t n);
 
 }  // namespace spj

@@ -170,7 +170,8 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
 }
 
 // ---------------------------------------------------------------------------
-// k_hist: digit histogram of one tile (glb_partition_count.c:119-371).
+// k_hist: digit histogram of one tile (glb_partition_count.c:119-371),
+// added into the counters of the tile's group (the tiles one XCD scatters).
 // For R with a heavy table it also records each heavy key's R rid.  For S
 // with a heavy table, heavy tuples are not counted (they never go to a
 // partition).  Key 0 is skipped everywhere.
@@ -213,18 +214,22 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
         }
     }
     __syncthreads();
+    // the tile's group counters (zeroed before the launch)
     uint32_t *dst = counts + td.cnt_off;
-    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) dst[i] = hist[i];
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x)
+        if (hist[i]) atomicAdd(&dst[i], hist[i]);
 }
 
 // ---------------------------------------------------------------------------
-// Column scan of tile counts into global write offsets
+// Column scan of group counts into global write offsets
 // (the page histogram + cross-rank prefix of queryHostFunc.cc:329-368,
-// 1574-1769, done on the device and without max-padding).
-//   a: per tile chunk, exclusive prefix down each digit column; chunk sums
+// 1574-1769, done on the device and without max-padding).  A segment's
+// "tiles" here are its scatter groups; the results are the groups' cursor
+// start offsets.
+//   a: per group chunk, exclusive prefix down each digit column; chunk sums
 //   b: per digit, exclusive prefix over chunk sums; digit totals
 //   c: per segment, exclusive prefix over digits -> partition starts
-//   d: add chunk base and partition start to every tile offset
+//   d: add chunk base and partition start to every group offset
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_scan_a(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
                                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ part)
@@ -293,68 +298,104 @@ __global__ __launch_bounds__(256) void k_scan_d(const ScanSeg *__restrict__ segs
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: LDS-staged radix scatter (glb_partition_packet.c:355-589 writes
-// one 8-byte tuple per mram_write; here a round of NT*IPT tuples is counting-
-// sorted by digit in LDS and each digit run is written with consecutive
-// lanes on consecutive addresses).  For S with a heavy table, heavy tuples
-// form bucket F and are emitted straight to the output as (R rid, S rid).
+// k_scatter: persistent LDS-staged radix scatter (glb_partition_packet.c:
+// 355-589 writes one 8-byte tuple per mram_write; here a round of NT*IPT
+// tuples is counting-sorted by digit in LDS and written digit run by digit
+// run).
+// Output space comes from cursors, one per (instance, relation, group,
+// digit): the scan leaves the start offset of group g's run of digit d in
+// counts[cnt_off + d], and each round reserves its run with one atomicAdd
+// per digit.  Group g is the tiles of XCD g's range, and block b runs on
+// XCD b % 8 and walks that range, so all workgroups of one XCD share the
+// cursors: their runs of a digit are adjacent, and the lines they write
+// partly fill up in that XCD's L2 instead of leaving half-written.  The
+// order of tuples inside a partition therefore depends on timing; the
+// partition contents do not.
+// For S with a heavy table, heavy tuples form bucket F and are emitted
+// straight to the output as (R rid, S rid).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void xcd_range(uint32_t x, uint32_t T, uint32_t &ts, uint32_t &te)
+{
+    const uint32_t q = T >> 3, r = T & 7u;
+    ts = x * q + min(x, r);
+    te = ts + q + (x < r ? 1u : 0u);
+}
+
 template <int IPT>
-__global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
-                                                 uint32_t ntiles, const uint32_t *__restrict__ counts,
-                                                 uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
+__device__ __forceinline__ void load_tile_round(uint2 *v, const InstDesc *__restrict__ insts, const TileDesc &td,
+                                                uint32_t o)
+{
+    constexpr uint32_t NT = kScatterNT;
+    const InstDesc &I = insts[td.inst];
+    const uint2 *in = td.rel ? I.S : I.R;
+    const uint32_t rn = min((uint32_t)(NT * IPT), td.end - o);
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t li = j * NT + threadIdx.x;
+        if (li < rn) v[j] = in[o + li];
+    }
+}
+
+template <int IPT>
+__global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restrict__ insts,
+                                                        const TileDesc *__restrict__ tiles, uint32_t ntiles,
+                                                        uint32_t max_bits, uint32_t *__restrict__ counts,
+                                                        uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
 {
     extern __shared__ uint32_t lds[];
-    constexpr uint32_t NT = 512, ROUND = NT * IPT;
+    constexpr uint32_t NT = kScatterNT, ROUND = NT * IPT;
     constexpr uint32_t DROP = 0xFFFFFFFFu;
-    const uint32_t t = xcd_tile(blockIdx.x, ntiles);
-    const TileDesc td = tiles[t];
-    const InstDesc &I = insts[td.inst];
-    const uint32_t F = 1u << I.bits, mask = F - 1u;
-    const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
+    constexpr uint32_t FMAX = 1u << kMaxBits;
+    uint32_t ts, te;
+    xcd_range(blockIdx.x & 7u, ntiles, ts, te);
+    const uint32_t W = gridDim.x >> 3;   // workgroups per XCD (grid is a multiple of 8)
+    uint32_t t = ts + (blockIdx.x >> 3);
+    if (t >= te) return;   // whole workgroup
+    const uint32_t FM = 1u << max_bits;
     // LDS layout
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
     uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
     uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
     uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
     uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
-    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // F + 2: counts -> starts
-    uint32_t *off = lst + F + 2;                                    // F running write offsets
-    uint32_t *bs = off + F;                                         // F: off[d] - start[d] this round
-    uint32_t *wtmp = bs + F;                                        // NT/64 + 1
+    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // FM + 2: counts -> starts
+    uint32_t *bs = lst + FM + 2;                                    // FM: write base - start, this round
+    uint32_t *wtmp = bs + FM;                                       // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
 
-    const uint32_t *cnt = counts + td.cnt_off;
-    for (uint32_t i = threadIdx.x; i < F; i += NT) off[i] = cnt[i];
-    for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-    if (heavy) load_heavy(I, hk, hr, hf);
-    const uint2 *in = td.rel ? I.S : I.R;
-    uint2 *outp = td.rel ? I.Sp : I.Rp;
-
-    // software pipeline: the next round's tuples are loaded while this round
-    // is ranked, staged and written
+    for (uint32_t i = threadIdx.x; i < FM + 2; i += NT) lst[i] = 0;
+    TileDesc td = tiles[t];
+    uint32_t o = td.begin;
+    uint32_t heavy_inst = DROP;   // instance whose heavy table is in LDS
+    // software pipeline: the next round's tuples (possibly of the next tile)
+    // are loaded while this round is ranked, staged and written
     uint2 nv[IPT];
-    {
-        const uint32_t rn = min(ROUND, td.end - td.begin);
-#pragma unroll
-        for (int j = 0; j < IPT; j++) {
-            const uint32_t li = j * NT + threadIdx.x;
-            if (li < rn) nv[j] = in[td.begin + li];
+    load_tile_round<IPT>(nv, insts, td, o);
+    for (;;) {
+        uint32_t tn = t, on = o + ROUND;
+        TileDesc tdn = td;
+        if (on >= td.end) {
+            tn = t + W;
+            if (tn < te) {
+                tdn = tiles[tn];
+                on = tdn.begin;
+            }
         }
-    }
-    __syncthreads();
-    for (uint32_t rb = td.begin; rb < td.end; rb += ROUND) {
-        const uint32_t rn = min(ROUND, td.end - rb);
+        const bool more = tn < te;
         uint2 v[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; j++) v[j] = nv[j];
-        if (rb + ROUND < td.end) {
-            const uint32_t nb = rb + ROUND, nn = min(ROUND, td.end - nb);
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                const uint32_t li = j * NT + threadIdx.x;
-                if (li < nn) nv[j] = in[nb + li];
-            }
+        if (more) load_tile_round<IPT>(nv, insts, tdn, on);
+
+        const InstDesc &I = insts[td.inst];
+        const uint32_t F = 1u << I.bits, mask = F - 1u;
+        const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
+        const uint32_t rn = min(ROUND, td.end - o);
+        __syncthreads();   // the previous round's write-out and clearing are done
+        if (heavy && heavy_inst != td.inst) {
+            load_heavy(I, hk, hr, hf);
+            heavy_inst = td.inst;
+            __syncthreads();
         }
         uint32_t dg[IPT], rk[IPT];
 #pragma unroll
@@ -387,6 +428,19 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
             if (dg[j] != DROP && dg[j] != F) rk[j] = atomicAdd(&lst[dg[j]], 1u);
         }
         __syncthreads();
+        // reserve each digit's run at the group cursor; the returned bases
+        // are only needed after the scan and staging, which hide the latency
+        uint32_t *cur = counts + td.cnt_off;
+        uint32_t cb[FMAX / NT];
+#pragma unroll
+        for (uint32_t u = 0; u < FMAX / NT; u++) {
+            const uint32_t d = u * NT + threadIdx.x;
+            cb[u] = 0;
+            if (d < F) {
+                const uint32_t c = lst[d];
+                if (c) cb[u] = atomicAdd(&cur[d], c);
+            }
+        }
         const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
         const uint32_t hstart = lst[F];                                  // == total without heavy tuples
 #pragma unroll
@@ -397,10 +451,10 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
                 sdig[p] = (uint16_t)dg[j];
             }
         }
-        for (uint32_t d = threadIdx.x; d < F; d += NT) {
-            const uint32_t s0 = lst[d], s1 = (d + 1 < F) ? lst[d + 1] : hstart;
-            bs[d] = off[d] - s0;
-            off[d] += s1 - s0;
+#pragma unroll
+        for (uint32_t u = 0; u < FMAX / NT; u++) {
+            const uint32_t d = u * NT + threadIdx.x;
+            if (d < F) bs[d] = cb[u] - lst[d];
         }
         if (heavy && threadIdx.x == 0) {
             // heavy pairs fill the output from its end downwards
@@ -408,233 +462,21 @@ __global__ __launch_bounds__(512) void k_scatter(const InstDesc *__restrict__ in
             misc[0] = hn ? I.nS - atomicAdd(&out_hi[td.inst], hn) - hn : 0u;
         }
         __syncthreads();
+        uint2 *outp = td.rel ? I.Sp : I.Rp;
         for (uint32_t i = threadIdx.x; i < hstart; i += NT) outp[bs[sdig[i]] + i] = stage[i];
         if (heavy) {
             const uint32_t hb = misc[0];
             for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
-                const uint32_t o = hb + (i - hstart);
-                if (o < I.nS) I.out[o] = stage[i];   // .x already holds the R rid
+                const uint32_t oi = hb + (i - hstart);
+                if (oi < I.nS) I.out[oi] = stage[i];   // .x already holds the R rid
                 else atomicOr(&err[td.inst], kErrOutOverflow);
             }
         }
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_scatter2: two-level radix scatter for large fan-out (bits >= 8).
-// With F = 4096 digits a round of 8K tuples leaves ~2 tuples per digit, so a
-// one-level scatter writes 16-byte runs and the memory system writes whole
-// lines for them.  Here a persistent workgroup takes a tile (up to 64K
-// tuples) and
-//   A. counting-sorts it by the top radix bits (coarse bucket, <= 64 of
-//      them) into its own scratch slot -- ~1 KB runs; the slot is reused for
-//      every tile, so it stays in L2 / the memory-side cache;
-//   B. scatters the coarse-sorted scratch by the full digit: a round of 8K
-//      consecutive scratch tuples spans few coarse buckets, so each digit
-//      gets a run of ~tile/F tuples (128 B at 64K / 4096).
-// Heavy S tuples are emitted as pairs in stage A, as in k_scatter.
-// ---------------------------------------------------------------------------
-template <int IPT>
-__global__ __launch_bounds__(512) void k_scatter2(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
-                                                  uint32_t ntiles, const uint32_t *__restrict__ counts,
-                                                  const uint32_t *__restrict__ pstart, uint2 *__restrict__ scratch,
-                                                  uint32_t scratch_per_wg, uint32_t *__restrict__ tile_ctr,
-                                                  uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
-{
-    extern __shared__ uint32_t lds[];
-    constexpr uint32_t NT = 512, ROUND = NT * IPT;
-    constexpr uint32_t DROP = 0xFFFFFFFFu;
-    constexpr uint32_t CMAX = 64;
-    uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
-    uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
-    uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
-    uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
-    uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
-    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // 4096 + 2
-    uint32_t *off = lst + 4096 + 2;                                 // 4096 running write offsets
-    uint32_t *bs = off + 4096;                                      // 4096
-    uint32_t *cs = bs + 4096;                                       // CMAX + 1 coarse starts in scratch
-    uint32_t *wtmp = cs + CMAX + 1;                                 // NT/64 + 1
-    uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy base, [1] tile
-    uint2 *scr = scratch + (size_t)blockIdx.x * scratch_per_wg;
-
-    for (;;) {
-        if (threadIdx.x == 0) misc[1] = atomicAdd(tile_ctr, 1u);
-        __syncthreads();
-        const uint32_t t = misc[1];
-        if (t >= ntiles) break;
-        const TileDesc td = tiles[t];
-        const InstDesc &I = insts[td.inst];
-        const uint32_t bits = I.bits, F = 1u << bits, mask = F - 1u;
-        const uint32_t cbits = bits / 2, fbits = bits - cbits, C = 1u << cbits;
-        const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
-        const uint2 *in = td.rel ? I.S : I.R;
-        uint2 *outp = td.rel ? I.Sp : I.Rp;
-        const uint32_t rel_len = td.rel ? I.nS : I.nR;
-        const uint32_t *cnt = counts + td.cnt_off;
-        const uint32_t *pst = pstart + (td.rel ? I.pstartS : I.pstartR);
-        // fine offsets of this tile, and its fine counts (next tile's offset,
-        // or the partition end for the relation's last tile)
-        for (uint32_t d = threadIdx.x; d < F; d += NT) {
-            const uint32_t o = cnt[d];
-            const uint32_t nx = (td.end == rel_len) ? pst[d + 1] : cnt[F + d];
-            off[d] = o;
-            bs[d] = nx - o;   // count, reused as a temporary
-        }
-        for (uint32_t i = threadIdx.x; i < C + 2; i += NT) lst[i] = 0;
-        if (heavy) load_heavy(I, hk, hr, hf);
-        __syncthreads();
-        // coarse bucket sizes -> starts in scratch
-        for (uint32_t c = threadIdx.x; c < C; c += NT) {
-            uint32_t sum = 0;
-            for (uint32_t f = 0; f < (1u << fbits); f++) sum += bs[(c << fbits) + f];
-            cs[c] = sum;
-        }
-        __syncthreads();
-        const uint32_t ntile = block_exclusive_scan(cs, C, wtmp);   // cs[c] = start of coarse bucket c
-        if (threadIdx.x == 0) cs[C] = ntile;
-
-        // ---- stage A: input -> scratch, grouped by coarse bucket
-        uint2 nv[IPT];
-        {
-            const uint32_t rn = min(ROUND, td.end - td.begin);
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                const uint32_t li = j * NT + threadIdx.x;
-                if (li < rn) nv[j] = in[td.begin + li];
-            }
-        }
-        __syncthreads();
-        for (uint32_t rb = td.begin; rb < td.end; rb += ROUND) {
-            const uint32_t rn = min(ROUND, td.end - rb);
-            uint2 v[IPT];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) v[j] = nv[j];
-            if (rb + ROUND < td.end) {
-                const uint32_t nb = rb + ROUND, nn = min(ROUND, td.end - nb);
-#pragma unroll
-                for (int j = 0; j < IPT; j++) {
-                    const uint32_t li = j * NT + threadIdx.x;
-                    if (li < nn) nv[j] = in[nb + li];
-                }
-            }
-            uint32_t dg[IPT], rk[IPT];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                const uint32_t li = j * NT + threadIdx.x;
-                dg[j] = DROP;
-                if (li < rn && v[j].x != kEmptyKey) {
-                    if (heavy) {
-                        const int s = heavy_find(hk, hf, v[j].x);
-                        if (s >= 0) {
-                            if (hr[s] != kNoRid) { dg[j] = C; v[j].x = hr[s]; }
-                        } else {
-                            dg[j] = (sfh(v[j].x) & mask) >> fbits;
-                        }
-                    } else {
-                        dg[j] = (sfh(v[j].x) & mask) >> fbits;
-                    }
-                }
-                if (heavy) {
-                    const uint64_t hm = __ballot(dg[j] == C);
-                    if (hm) {
-                        uint32_t hb = 0;
-                        if (lane_id() == 0) hb = atomicAdd(&lst[C], (uint32_t)__popcll(hm));
-                        hb = __shfl(hb, 0, kWave);
-                        if (dg[j] == C) rk[j] = hb + mbcnt(hm);
-                    }
-                }
-                if (dg[j] != DROP && dg[j] != C) rk[j] = atomicAdd(&lst[dg[j]], 1u);
-            }
-            __syncthreads();
-            const uint32_t total = block_exclusive_scan(lst, C + 1, wtmp);
-            const uint32_t hstart = lst[C];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                if (dg[j] != DROP) {
-                    const uint32_t p = lst[dg[j]] + rk[j];
-                    stage[p] = v[j];
-                    sdig[p] = (uint16_t)dg[j];
-                }
-            }
-            for (uint32_t c = threadIdx.x; c < C; c += NT) {
-                const uint32_t s0 = lst[c], s1 = (c + 1 < C) ? lst[c + 1] : hstart;
-                bs[c] = cs[c] - s0;   // scratch position = bs[c] + staged index
-                cs[c] += s1 - s0;
-            }
-            if (heavy && threadIdx.x == 0) {
-                const uint32_t hn = total - hstart;
-                misc[0] = hn ? I.nS - atomicAdd(&out_hi[td.inst], hn) - hn : 0u;
-            }
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < hstart; i += NT) scr[bs[sdig[i]] + i] = stage[i];
-            if (heavy) {
-                const uint32_t hb = misc[0];
-                for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
-                    const uint32_t o = hb + (i - hstart);
-                    if (o < I.nS) I.out[o] = stage[i];
-                    else atomicOr(&err[td.inst], kErrOutOverflow);
-                }
-            }
-            for (uint32_t i = threadIdx.x; i < C + 2; i += NT) lst[i] = 0;
-            __syncthreads();
-        }
-        // scratch writes must be visible to the whole workgroup before stage B
-        __threadfence_block();
-        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-        __syncthreads();
-
-        // ---- stage B: scratch -> partitions, by full digit
-        {
-            const uint32_t rn = min(ROUND, ntile);
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                const uint32_t li = j * NT + threadIdx.x;
-                if (li < rn) nv[j] = scr[li];
-            }
-        }
-        for (uint32_t rb = 0; rb < ntile; rb += ROUND) {
-            const uint32_t rn = min(ROUND, ntile - rb);
-            uint2 v[IPT];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) v[j] = nv[j];
-            if (rb + ROUND < ntile) {
-                const uint32_t nb = rb + ROUND, nn = min(ROUND, ntile - nb);
-#pragma unroll
-                for (int j = 0; j < IPT; j++) {
-                    const uint32_t li = j * NT + threadIdx.x;
-                    if (li < nn) nv[j] = scr[nb + li];
-                }
-            }
-            uint32_t dg[IPT], rk[IPT];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                const uint32_t li = j * NT + threadIdx.x;
-                dg[j] = (li < rn) ? (sfh(v[j].x) & mask) : DROP;
-                if (dg[j] != DROP) rk[j] = atomicAdd(&lst[dg[j]], 1u);
-            }
-            __syncthreads();
-            (void)block_exclusive_scan(lst, F, wtmp);
-#pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                if (dg[j] != DROP) {
-                    const uint32_t p = lst[dg[j]] + rk[j];
-                    stage[p] = v[j];
-                    sdig[p] = (uint16_t)dg[j];
-                }
-            }
-            for (uint32_t d = threadIdx.x; d < F; d += NT) {
-                const uint32_t s0 = lst[d], s1 = (d + 1 < F) ? lst[d + 1] : rn;
-                bs[d] = off[d] - s0;
-                off[d] += s1 - s0;
-            }
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < rn; i += NT) outp[bs[sdig[i]] + i] = stage[i];
-            for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
-            __syncthreads();
-        }
+        if (!more) break;
+        t = tn;
+        td = tdn;
+        o = on;
     }
 }
 
@@ -1215,18 +1057,11 @@ size_t lds_hist(uint32_t max_bits)
     return ((size_t(1) << max_bits) + 2 * kHeavySlots + (1u << kHeavyFilterLog2) / 32) * 4;
 }
 
-size_t lds_scatter2(int ipt)
-{
-    const size_t R = 512u * (size_t)ipt;
-    return R * 8 + R * 2 + (4096 + 2) * 4 + 2 * 4096 * 4 + (64 + 1) * 4 + (512 / kWave + 1) * 4 + 16 +
-           2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8;
-}
-
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
-    const size_t F = size_t(1) << max_bits, R = 512u * (size_t)ipt;
-    return R * 8 + R * 2 + (F + 2) * 4 + 2 * F * 4 + (512 / kWave + 1) * 4 + 16 + 2 * kHeavySlots * 4 +
-           (1u << kHeavyFilterLog2) / 8;
+    const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
+    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + R * 2 + (F + 2) * 4 + F * 4 +
+           (kScatterNT / kWave + 1) * 4 + 16;
 }
 
 size_t lds_join()
@@ -1275,34 +1110,24 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st)
+hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
+                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, uint32_t grid, hipStream_t st)
 {
     if (!ntiles) return hipSuccess;
+    grid = (grid + 7u) & ~7u;   // whole XCD groups
     size_t l = lds_scatter(max_bits, ipt);
     hipError_t e;
-    if (ipt == 16) {
-        e = set_lds((const void *)k_scatter<16>, l);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_scatter<16>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_hi, err);
-    } else {
+    if (ipt == 8) {
         e = set_lds((const void *)k_scatter<8>, l);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_scatter<8>, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts, out_hi, err);
+        hipLaunchKernelGGL(k_scatter<8>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
+                           out_hi, err);
+    } else {
+        e = set_lds((const void *)k_scatter<4>, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter<4>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
+                           out_hi, err);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                           const uint32_t *pstart, uint2 *scratch, uint32_t scratch_per_wg, uint32_t grid,
-                           uint32_t *tile_ctr, uint32_t *out_hi, uint32_t *err, hipStream_t st)
-{
-    if (!ntiles) return hipSuccess;
-    size_t l = lds_scatter2(16);
-    hipError_t e = set_lds((const void *)k_scatter2<16>, l);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter2<16>, dim3(grid), dim3(512), l, st, insts, tiles, ntiles, counts, pstart, scratch,
-                       scratch_per_wg, tile_ctr, out_hi, err);
     return hipGetLastError();
 }
 

@@ -20,14 +20,10 @@ hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t nt
 hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk_ad, uint32_t nblk_ad,
                        const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
                        uint32_t max_bits, hipStream_t st);
-hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, hipStream_t st);
-// Two-level scatter (radix bits 8..12), persistent over `grid` workgroups
-// with one scratch slot of scratch_per_wg tuples each.
-hipError_t launch_scatter2(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, const uint32_t *counts,
-                           const uint32_t *pstart, uint2 *scratch, uint32_t scratch_per_wg, uint32_t grid,
-                           uint32_t *tile_ctr, uint32_t *out_hi, uint32_t *err, hipStream_t st);
-size_t lds_scatter2(int ipt);
+// Persistent scatter over `grid` workgroups (rounded up to a multiple of 8);
+// counts holds the scanned group cursors and is advanced in place.
+hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
+                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, uint32_t grid, hipStream_t st);
 // Sample blocks for heavy detection of an S of nS tuples (at most
 // max_blocks, evenly strided); *stride receives the stride in blocks.
 uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride);

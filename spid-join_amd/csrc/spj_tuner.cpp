@@ -168,7 +168,7 @@ int spj_tuner_load(const char *path)
 
 /* Time candidate configurations for the bucket of (R, S, zipf) on the given
  * device-resident sample relations and keep the fastest.  Candidates vary
- * radix bits around the default (-2..+1), scatter items per thread (8, 16)
+ * radix bits around the default (-2..+1), scatter tuples per thread (4, 8)
  * and LDS table bucket count (2^11, 2^12).  reps timed runs per candidate (median).
  * best_ms receives the winner's median time. */
 int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out, double zipf,
@@ -181,16 +181,12 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
     for (int db = -2; db <= 1; db++) {
         int b = (int)base.bits + db;
         if (b < 0 || b > (int)kMaxBits) continue;
-        for (uint32_t ipt : {8u, 16u}) {
+        for (uint32_t ipt : {4u, 8u}) {
             for (uint32_t tl : {11u, 12u}) {
                 Resolved r = base;
                 r.bits = (uint32_t)b;
                 r.ipt = ipt;
                 r.table_log2 = tl;
-                // keep the counter array bounded for the new bit count
-                uint64_t n = nR > nS ? nR : nS, t = 8192;
-                while ((n / t) * (uint64_t(1) << r.bits) > (uint64_t(1) << 24)) t <<= 1;
-                r.tile = (uint32_t)t;
                 // R partitions must fit the LDS table in at most a few sub-tables
                 uint64_t per = nR >> r.bits;
                 if (per > 4ull * join_table_cap()) continue;

@@ -128,13 +128,14 @@ def test_join_instance_lifecycle(gpu, orc):
     ji.close()
 
 
-def test_batch_mixed_two_level(gpu, orc):
-    """A batch where one instance uses the two-level scatter (>= 256
-    partitions) and heavy keys, next to small one-level instances."""
+def test_batch_mixed_sizes(gpu, orc):
+    """A batch where one instance has 256 partitions and heavy keys, next to
+    small instances: tiles of different instances share scatter workgroups
+    and XCD groups."""
     sj = gpu
     Rs = [sj.gen_r(1 << 22, 51), sj.gen_r(5000, 52), sj.gen_r(70000, 53)]
     Ss = [sj.gen_s(1 << 23, Rs[0], 1.5, 61), sj.gen_s(20000, Rs[1], 0.0, 62), sj.gen_s(280000, Rs[2], 1.0, 63)]
-    print("case batch two-level", flush=True)
+    print("case batch mixed sizes", flush=True)
     outs, _ = sj.join_batch(Rs, Ss)
     for R, S, p in zip(Rs, Ss, outs):
         check_equal(sj, orc, R, S, p)
