@@ -49,7 +49,7 @@ enum { SPJ_ALG_AUTO = -1, SPJ_ALG_PHJ = 0, SPJ_ALG_NPHJ = 1, SPJ_ALG_SMJ = 2, SP
 typedef struct {
     int32_t algorithm;        /* SPJ_ALG_*: AUTO = hash, nested loop for tiny partitions */
     int32_t radix_bits;       /* partition-pass radix bits (0..12) */
-    int32_t table_log2;       /* LDS hash-table buckets, log2 (6..12; 12 = the reference's 4096) */
+    int32_t table_log2;       /* LDS hash-table buckets, log2 (6..13; 12 = the reference's 4096) */
     int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4 or 8 */

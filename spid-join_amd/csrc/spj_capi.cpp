@@ -63,7 +63,7 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
 int check_resolved(const Resolved &r)
 {
     if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
-    if (r.table_log2 < 6 || r.table_log2 > 12) return fail(SPJ_EINVAL, "table_log2 %u outside 6..12", r.table_log2);
+    if (r.table_log2 < 6 || r.table_log2 > 13) return fail(SPJ_EINVAL, "table_log2 %u outside 6..13", r.table_log2);
     if (!(r.alg == 0 || r.alg == 2 || r.alg == 3)) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
     if (!(r.ipt == 4 || r.ipt == 8)) return fail(SPJ_EINVAL, "scatter_ipt %u (4 or 8)", r.ipt);
     if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);

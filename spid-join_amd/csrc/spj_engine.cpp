@@ -85,7 +85,7 @@ uint32_t default_tile(uint64_t n)
 Resolved default_params(uint32_t nR, uint32_t nS)
 {
     Resolved r;
-    r.table_log2 = 12;   // 4096 buckets, the reference table size
+    r.table_log2 = 13;   // 8192 buckets: about 2 R tuples per bucket at 16K tuples
     // Aim at <= 16K R tuples per partition: one LDS table (17408 tuples) with
     // room for the binomial spread of partition sizes.
     uint32_t b = nR > 16384 ? ceil_log2((uint64_t(nR) + 16383) / 16384) : 0;

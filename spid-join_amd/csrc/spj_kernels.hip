@@ -431,6 +431,11 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // reserve each digit's run at the group cursor; the returned bases
         // are only needed after the scan and staging, which hide the latency
         uint32_t *cur = counts + td.cnt_off;
+        uint32_t hres = 0;   // heavy pairs: reserved from the back of the output
+        if (heavy && threadIdx.x == 0) {
+            const uint32_t hn = lst[F];
+            if (hn) hres = atomicAdd(&out_hi[td.inst], hn);
+        }
         uint32_t cb[FMAX / NT];
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         if (heavy && threadIdx.x == 0) {
             // heavy pairs fill the output from its end downwards
             const uint32_t hn = total - hstart;
-            misc[0] = hn ? I.nS - atomicAdd(&out_hi[td.inst], hn) - hn : 0u;
+            misc[0] = hn ? I.nS - hres - hn : 0u;
         }
         __syncthreads();
         uint2 *outp = td.rel ? I.Sp : I.Rp;
@@ -633,11 +638,11 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 // k_join: persistent workgroups pop items from the queue.  Per item the
 // algorithm is chosen by partition size and the instance's setting:
 //   hash (a2a_srj_build.c + a2a_srj_probe_inner_mt.c): R_p is counting-
-//     sorted into an LDS table of 4096 buckets (the reference's table has
-//     4096 slots) by a multiply-shift hash of the key; a probe reads one
-//     packed bucket word and scans the bucket (first equal key wins, as the
-//     reference's linear probe).  Load factor 1: up to kTableCap R tuples per
-//     workgroup.  A larger R_p is split into Q sub-tables by high hash bits;
+//     sorted into an LDS table of up to 8192 buckets (the reference's table
+//     has 4096 slots) by a multiply-shift hash of the key; a probe reads two
+//     16-bit bucket ends and scans the bucket (first equal key wins, as the
+//     reference's linear probe).  Up to kTableCap R tuples per workgroup.
+//     A larger R_p is split into Q sub-tables by high hash bits;
 //   nested loop (nested_loop_join.c) for tiny R_p;
 //   sort-merge (mpsm_join_sort_probe.c): R_p and S chunks bitonic-sorted in
 //     LDS, then a per-thread merge.
@@ -649,9 +654,10 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 constexpr uint32_t kJoinNT = 1024;
 constexpr uint32_t kJoinIPT = 8;
 constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
-constexpr uint32_t kBuckets = 4096;         // at most 4096 buckets (the reference's 4096 slots)
-constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket words
-constexpr uint32_t kScan = 8;               // bucket entries read unconditionally per probe
+constexpr uint32_t kBucketBits = 13;
+constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-bit ends: 16 KB)
+constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
+constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
 constexpr uint32_t kSortS = 2048;           // sort-merge: S chunk sorted per merge
 constexpr uint32_t kNLChunk = 8192;
@@ -677,6 +683,35 @@ __device__ __forceinline__ void wave_emit(bool hit, uint32_t r_rid, uint32_t s_r
         const uint32_t pos = wb + mbcnt(m);
         if (pos < sh->res) out[sh->base + pos] = make_uint2(r_rid, s_rid);
         else atomicOr(err_i, kErrOutOverflow);
+    }
+}
+
+// Append the hits of a whole round (bit j of hitm: pair p[j] of this lane
+// is a hit): one LDS atomic per wave for all N probes.
+template <int N>
+__device__ __forceinline__ void wave_emit_round(uint32_t hitm, const uint2 *p, JoinShared *sh, uint2 *out,
+                                                uint32_t *err_i)
+{
+    uint64_t m[N];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        m[j] = __ballot((hitm >> j) & 1u);
+        tot += (uint32_t)__popcll(m[j]);
+    }
+    if (tot == 0) return;
+    uint32_t wb = 0;
+    if (lane_id() == 0) wb = atomicAdd(&sh->fill, tot);
+    wb = __shfl(wb, 0, kWave);
+    const uint32_t res = sh->res, base = sh->base;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        if ((hitm >> j) & 1u) {
+            const uint32_t pos = wb + mbcnt(m[j]);
+            if (pos < res) out[base + pos] = p[j];
+            else atomicOr(err_i, kErrOutOverflow);
+        }
+        wb += (uint32_t)__popcll(m[j]);
     }
 }
 
@@ -852,104 +887,95 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             }
         } else {
             // ---------------- hash join
-            // LDS: (key, rid) pairs counting-sorted by bucket, then one packed
-            // word per bucket: start << 16 | size (both < 2^16 = kTableCap bound)
+            // LDS: (key, rid) pairs counting-sorted by bucket, then one 16-bit
+            // end offset per bucket: bucket b is [bend[b - 1], bend[b]).
+            // During the count pass the 32-bit bucket counters live in the
+            // pair area, which the fill pass overwrites afterwards.
             uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
-            uint32_t *be = lds + 2 * (kTableCap + kScan);
+            uint32_t *bcnt = lds;                           // nb counters (count pass only)
+            uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets
+            uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
             const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
-            const uint32_t lb = min(I.table_log2, 12u), nb = 1u << lb;   // buckets
+            const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
+            constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
             for (uint32_t q = 0; q < Q; q++) {
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < nb; i += NT) be[i] = 0;
+                for (uint32_t i = threadIdx.x; i < nb; i += NT) bcnt[i] = 0;
                 __syncthreads();
-                uint32_t ntab = 0;   // entries in this table
+                // R_p fits one table (Q == 1): hold it in registers (<= RPT
+                // tuples per thread) and read it once; otherwise each
+                // sub-table reads R_p twice, keeping its own keys
+                uint2 r[RPT];
+#pragma unroll
+                for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
                 if (Q == 1) {
-                    // R_p fits one table: hold it in registers (<= 17 tuples per
-                    // thread), count buckets, scan, then place -- one read of R_p
-                    constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
-                    uint2 r[RPT];
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = rb + u * NT + threadIdx.x;
-                        r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                        if (i < re) r[u] = I.Rp[i];
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++)
-                        if (r[u].x != kEmptyKey) atomicAdd(&be[bucket_of(r[u].x, lb)], 1u);
-                    __syncthreads();
-                    ntab = block_exclusive_scan(be, nb, wtmp);
+                        if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bucket_of(r[u].x, lb)], 1u);
+                } else {
+                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                        uint2 c[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const uint32_t i = i0 + u * NT + threadIdx.x;
+                            c[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                            atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
+                        }
+                    }
+                }
+                __syncthreads();
+                const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                if (ntab > kTableCap) {
+                    if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
+                    continue;   // uniform: ntab is the same in every thread
+                }
+                // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap)
+                for (uint32_t i = threadIdx.x; i < nb; i += NT) bend[i] = (uint16_t)bcnt[i];
+                __syncthreads();
+                // fill pass: a packed atomic advances one 16-bit cursor of a
+                // word; it never carries, every cursor staying <= ntab
+                if (Q == 1) {
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         if (r[u].x == kEmptyKey) continue;
-                        tkv[atomicAdd(&be[bucket_of(r[u].x, lb)], 1u)] = r[u];
+                        const uint32_t bk = bucket_of(r[u].x, lb), sh16 = (bk & 1u) * 16u;
+                        const uint32_t pos = (atomicAdd(&bend_w[bk >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                        tkv[pos] = r[u];
                     }
-                    __syncthreads();
                 } else {
-                    // count pass: tuples per bucket
                     for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                        uint2 r[4];
+                        uint2 c[4];
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const uint32_t i = i0 + u * NT + threadIdx.x;
-                            r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                            c[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
                         }
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
-                            if (r[u].x == kEmptyKey || sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
-                            atomicAdd(&be[bucket_of(r[u].x, lb)], 1u);
+                            if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                            const uint32_t bk = bucket_of(c[u].x, lb), sh16 = (bk & 1u) * 16u;
+                            const uint32_t pos = (atomicAdd(&bend_w[bk >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                            tkv[pos] = c[u];
                         }
                     }
-                    __syncthreads();
-                    const uint32_t nq = block_exclusive_scan(be, nb, wtmp);   // be[b] = start of bucket b
-                    ntab = nq;
-                    if (nq > kTableCap) {
-                        if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
-                        continue;   // uniform: nq is the same in every thread
-                    }
-                    // fill pass: be[b] advances to the end of bucket b
-                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                        uint2 r[4];
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const uint32_t i = i0 + u * NT + threadIdx.x;
-                            r[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
-                        }
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            if (r[u].x == kEmptyKey || sub_of(sfh(r[u].x), I.bits, Q) != q) continue;
-                            tkv[atomicAdd(&be[bucket_of(r[u].x, lb)], 1u)] = r[u];
-                        }
-                    }
-                    __syncthreads();
                 }
-                // be[b] holds the end of bucket b: pack start << 16 | size
-                {
-                    uint32_t pk[(4096 + NT - 1) / NT];
-#pragma unroll
-                    for (uint32_t u = 0; u < (4096 + NT - 1) / NT; u++) {
-                        const uint32_t b = u * NT + threadIdx.x;
-                        if (b < nb) {
-                            const uint32_t st = b ? be[b - 1] : 0u, en = be[b];
-                            pk[u] = (st << 16) | (en - st);
-                        }
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (uint32_t u = 0; u < (4096 + NT - 1) / NT; u++) {
-                        const uint32_t b = u * NT + threadIdx.x;
-                        if (b < nb) be[b] = pk[u];
-                    }
-                    __syncthreads();
-                }
+                __syncthreads();   // bend[b] = end of bucket b
                 // duplicate R keys (a2a_srj_build.c:265-271): each entry against
                 // the later entries of its bucket
                 {
                     bool dup = false;
                     for (uint32_t i = threadIdx.x; i < ntab; i += NT) {
                         const uint32_t k = tkv[i].x;
-                        if (k == kEmptyKey) continue;
-                        const uint32_t d = be[bucket_of(k, lb)];
-                        const uint32_t en = (d >> 16) + (d & 0xFFFFu);
+                        const uint32_t en = bend[bucket_of(k, lb)];
                         for (uint32_t j = i + 1; j < en; j++) dup |= (tkv[j].x == k);
                     }
                     if (dup) atomicOr(erri, kErrDuplicateKey);
@@ -962,17 +988,18 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
                     if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, s0 + ROUND, w.se);
+                    uint32_t hitm = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
                         const uint32_t k = v[j].x;
                         bool hit = false;
                         uint32_t rr = 0;
                         if (k != kEmptyKey && (Q == 1 || sub_of(sfh(k), I.bits, Q) == q)) {
-                            const uint32_t d = be[bucket_of(k, lb)];
-                            const uint32_t st = d >> 16, n = d & 0xFFFFu;
-                            // first kScan entries without a data-dependent loop (the
-                            // table has kScan padding slots); a longer bucket goes on
-                            // in the tail loop
+                            const uint32_t bk = bucket_of(k, lb);
+                            const uint32_t st = bk ? bend[bk - 1] : 0u, n = bend[bk] - st;
+                            // the first kScan entries without a data-dependent loop
+                            // (the table has kScan padding slots); a longer bucket
+                            // goes on in the tail loop
 #pragma unroll
                             for (uint32_t u = 0; u < kScan; u++) {
                                 const uint2 e = tkv[st + u];
@@ -983,8 +1010,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                 if (e.x == k) { rr = e.y; hit = true; }
                             }
                         }
-                        wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
+                        v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
+                        hitm |= (hit ? 1u : 0u) << j;
                     }
+                    wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
                 }
             }
         }
@@ -1066,7 +1095,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 
 size_t lds_join()
 {
-    size_t hash = (2 * ((size_t)kTableCap + kScan) + kBuckets) * 4;
+    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + kBuckets * 2;
     size_t smj = ((size_t)kSortR + kSortS) * 8 + kJoinNT * 4;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
