@@ -202,15 +202,16 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
             const uint32_t i = b + j * blockDim.x + threadIdx.x;
             v[j] = (i < td.end) ? in[i] : make_uint2(kEmptyKey, 0u);
         }
+        // heavy lookups of all U tuples first, so their LDS reads overlap
+        int hs[U];
+#pragma unroll
+        for (uint32_t j = 0; j < U; j++) hs[j] = (mode != 0 && v[j].x != kEmptyKey) ? heavy_find(hk, hf, v[j].x) : -1;
 #pragma unroll
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
-            if (mode == 2 && heavy_find(hk, hf, v[j].x) >= 0) continue;   // heavy S tuple: no partition
+            if (mode == 2 && hs[j] >= 0) continue;   // heavy S tuple: no partition
             atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
-            if (mode == 1) {
-                const int s = heavy_find(hk, hf, v[j].x);
-                if (s >= 0) I.heavy_rids[s] = v[j].y;   // R keys are unique: one writer per slot
-            }
+            if (mode == 1 && hs[j] >= 0) I.heavy_rids[hs[j]] = v[j].y;   // R keys are unique: one writer per slot
         }
     }
     __syncthreads();
@@ -398,35 +399,53 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             __syncthreads();
         }
         uint32_t dg[IPT], rk[IPT];
+        if (heavy) {
+            // in phases, so that the LDS reads of all IPT tuples overlap:
+            // table lookups, then R rids, then one rank atomic per wave for
+            // the heavy bucket F, then the digit atomics below
+            int hs[IPT];
 #pragma unroll
-        for (int j = 0; j < IPT; j++) {
-            const uint32_t li = j * NT + threadIdx.x;
-            dg[j] = DROP;
-            if (li < rn && v[j].x != kEmptyKey) {
-                if (heavy) {
-                    const int s = heavy_find(hk, hf, v[j].x);
-                    if (s >= 0) {
-                        // heavy: bucket F, unless the key has no R tuple
-                        if (hr[s] != kNoRid) { dg[j] = F; v[j].x = hr[s]; }
-                    } else {
-                        dg[j] = sfh(v[j].x) & mask;
-                    }
-                } else {
-                    dg[j] = sfh(v[j].x) & mask;
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, hf, v[j].x) : -1;
+            }
+            uint32_t hrid[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : kNoRid;
+            uint64_t hm[IPT];
+            uint32_t htot = 0;
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                dg[j] = DROP;
+                if (li < rn && v[j].x != kEmptyKey) {
+                    if (hs[j] < 0) dg[j] = sfh(v[j].x) & mask;
+                    else if (hrid[j] != kNoRid) { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
+                    // (a heavy key with no R tuple is dropped)
+                }
+                hm[j] = __ballot(dg[j] == F);
+                htot += (uint32_t)__popcll(hm[j]);
+            }
+            if (htot) {
+                uint32_t hb = 0;
+                if (lane_id() == 0) hb = atomicAdd(&lst[F], htot);
+                hb = __shfl(hb, 0, kWave);
+#pragma unroll
+                for (int j = 0; j < IPT; j++) {
+                    if (dg[j] == F) rk[j] = hb + mbcnt(hm[j]);
+                    hb += (uint32_t)__popcll(hm[j]);
                 }
             }
-            if (heavy) {
-                // heavy tuples all share bucket F: one LDS atomic per wave
-                const uint64_t hm = __ballot(dg[j] == F);
-                if (hm) {
-                    uint32_t hb = 0;
-                    if (lane_id() == 0) hb = atomicAdd(&lst[F], (uint32_t)__popcll(hm));
-                    hb = __shfl(hb, 0, kWave);
-                    if (dg[j] == F) rk[j] = hb + mbcnt(hm);
-                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < IPT; j++) {
+                const uint32_t li = j * NT + threadIdx.x;
+                dg[j] = (li < rn && v[j].x != kEmptyKey) ? (sfh(v[j].x) & mask) : DROP;
             }
-            if (dg[j] != DROP && dg[j] != F) rk[j] = atomicAdd(&lst[dg[j]], 1u);
         }
+#pragma unroll
+        for (int j = 0; j < IPT; j++)
+            if (dg[j] != DROP && dg[j] != F) rk[j] = atomicAdd(&lst[dg[j]], 1u);
         __syncthreads();
         // reserve each digit's run at the group cursor; the returned bases
         // are only needed after the scan and staging, which hide the latency
