@@ -52,7 +52,7 @@ typedef struct {
     int32_t table_log2;       /* LDS hash-table buckets, log2 (6..13; 12 = the reference's 4096) */
     int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
-    int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4 or 8 */
+    int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
     int64_t tile_tuples;      /* tuples per partition-pass tile */
     int64_t items_s_max;      /* S tuples per join work item */
 } spj_config_t;

@@ -65,7 +65,8 @@ int check_resolved(const Resolved &r)
     if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
     if (r.table_log2 < 6 || r.table_log2 > 13) return fail(SPJ_EINVAL, "table_log2 %u outside 6..13", r.table_log2);
     if (!(r.alg == 0 || r.alg == 2 || r.alg == 3)) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
-    if (!(r.ipt == 4 || r.ipt == 8)) return fail(SPJ_EINVAL, "scatter_ipt %u (4 or 8)", r.ipt);
+    if (!(r.ipt == 4 || r.ipt == 8 || r.ipt == 10 || r.ipt == 12))
+        return fail(SPJ_EINVAL, "scatter_ipt %u (4, 8, 10 or 12)", r.ipt);
     if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);
     if (r.smax < 512) return fail(SPJ_EINVAL, "items_s_max %u < 512", r.smax);
     return SPJ_OK;

@@ -90,7 +90,7 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     // room for the binomial spread of partition sizes.
     uint32_t b = nR > 16384 ? ceil_log2((uint64_t(nR) + 16383) / 16384) : 0;
     r.bits = std::min<uint32_t>(b, kMaxBits);
-    r.ipt = 8;
+    r.ipt = 12;
     r.tile = default_tile(std::max(nR, nS));
     r.smax = 65536;
     r.heavy = nS >= (1u << 20) ? 1u : 0u;

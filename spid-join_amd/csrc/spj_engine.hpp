@@ -19,7 +19,7 @@ struct Resolved {
     uint32_t alg = 0;         // 0 hash (+NL for tiny), 2 sort-merge, 3 nested loop
     uint32_t heavy = 0;       // heavy-key detection and direct emission
     uint32_t heavy_min = 16;  // summed sample count for a key to be heavy
-    uint32_t ipt = 8;         // scatter tuples per thread (round = 1024 * ipt)
+    uint32_t ipt = 12;        // scatter tuples per thread (round = 1024 * ipt)
     uint32_t tile = 65536;    // tuples per tile of the partition pass
     uint32_t smax = 65536;    // S tuples per work item
 };

@@ -358,8 +358,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
     uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
     uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
-    uint16_t *sdig = reinterpret_cast<uint16_t *>(stage + ROUND);   // ROUND digits
-    uint32_t *lst = reinterpret_cast<uint32_t *>(sdig + ROUND);     // FM + 2: counts -> starts
+    uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
     uint32_t *bs = lst + FM + 2;                                    // FM: write base - start, this round
     uint32_t *wtmp = bs + FM;                                       // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
@@ -472,7 +471,6 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             if (dg[j] != DROP) {
                 const uint32_t p = lst[dg[j]] + rk[j];
                 stage[p] = v[j];
-                sdig[p] = (uint16_t)dg[j];
             }
         }
 #pragma unroll
@@ -487,7 +485,12 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         }
         __syncthreads();
         uint2 *outp = td.rel ? I.Sp : I.Rp;
-        for (uint32_t i = threadIdx.x; i < hstart; i += NT) outp[bs[sdig[i]] + i] = stage[i];
+        // a staged tuple's digit is recomputed from its key (cheaper than
+        // staging it); heavy tuples, whose key was replaced, lie beyond hstart
+        for (uint32_t i = threadIdx.x; i < hstart; i += NT) {
+            const uint2 t = stage[i];
+            outp[bs[sfh(t.x) & mask] + i] = t;
+        }
         if (heavy) {
             const uint32_t hb = misc[0];
             for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
@@ -1108,7 +1111,7 @@ size_t lds_hist(uint32_t max_bits)
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
-    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + R * 2 + (F + 2) * 4 + F * 4 +
+    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + (F + 2) * 4 + F * 4 +
            (kScatterNT / kWave + 1) * 4 + 16;
 }
 
@@ -1165,7 +1168,17 @@ hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t
     grid = (grid + 7u) & ~7u;   // whole XCD groups
     size_t l = lds_scatter(max_bits, ipt);
     hipError_t e;
-    if (ipt == 8) {
+    if (ipt == 12) {
+        e = set_lds((const void *)k_scatter<12>, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter<12>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
+                           out_hi, err);
+    } else if (ipt == 10) {
+        e = set_lds((const void *)k_scatter<10>, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter<10>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
+                           out_hi, err);
+    } else if (ipt == 8) {
         e = set_lds((const void *)k_scatter<8>, l);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<8>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,

@@ -168,7 +168,7 @@ int spj_tuner_load(const char *path)
 
 /* Time candidate configurations for the bucket of (R, S, zipf) on the given
  * device-resident sample relations and keep the fastest.  Candidates vary
- * radix bits around the default (-2..+1), scatter tuples per thread (4, 8)
+ * radix bits around the default (-2..+1), scatter tuples per thread (8, 12)
  * and LDS table bucket count (2^12, 2^13).  reps timed runs per candidate (median).
  * best_ms receives the winner's median time. */
 int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out, double zipf,
@@ -181,7 +181,7 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
     for (int db = -2; db <= 1; db++) {
         int b = (int)base.bits + db;
         if (b < 0 || b > (int)kMaxBits) continue;
-        for (uint32_t ipt : {4u, 8u}) {
+        for (uint32_t ipt : {8u, 12u}) {
             for (uint32_t tl : {12u, 13u}) {
                 Resolved r = base;
                 r.bits = (uint32_t)b;
