@@ -92,7 +92,7 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     r.bits = std::min<uint32_t>(b, kMaxBits);
     r.ipt = 12;
     r.tile = default_tile(std::max(nR, nS));
-    r.smax = 65536;
+    r.smax = 131072;   // at 16K R tuples per partition: several S tuples per table build
     r.heavy = nS >= (1u << 20) ? 1u : 0u;
     r.heavy_min = 16;
     r.alg = 0;
@@ -140,7 +140,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     // ---- plan: regions, tiles, scan segments
     uint64_t rp_total = 0, sp_total = 0, cnt_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
     // one S-range size per batch (the item kernel takes a single value)
-    const uint32_t smax = specs.empty() ? 65536u : specs[0].cfg.smax;
+    const uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
     uint32_t nheavy_inst = 0;
     std::vector<uint64_t> rp_off(n), sp_off(n);
     std::vector<int> heavy_idx(n, -1);

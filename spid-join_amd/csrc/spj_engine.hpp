@@ -21,7 +21,7 @@ struct Resolved {
     uint32_t heavy_min = 16;  // summed sample count for a key to be heavy
     uint32_t ipt = 12;        // scatter tuples per thread (round = 1024 * ipt)
     uint32_t tile = 65536;    // tuples per tile of the partition pass
-    uint32_t smax = 65536;    // S tuples per work item
+    uint32_t smax = 131072;   // S tuples per work item
 };
 
 struct InstSpec {

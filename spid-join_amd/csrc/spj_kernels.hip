@@ -23,6 +23,36 @@
 namespace spj {
 
 // ---------------------------------------------------------------------------
+// Accesses through the device pointers held in InstDesc.  Those pointers are
+// generic, and generic (FLAT) instructions count on lgkmcnt as well as on
+// vmcnt, so every LDS wait and barrier would also wait for the streaming
+// loads and stores in flight.  These helpers access them as global memory.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) uint64_t gu64_t;
+typedef __attribute__((address_space(1))) uint32_t gu32_t;
+
+__device__ __forceinline__ uint2 gld(const uint2 *p, uint64_t i)
+{
+    const uint64_t x = reinterpret_cast<const gu64_t *>(reinterpret_cast<uintptr_t>(p))[i];
+    return make_uint2((uint32_t)x, (uint32_t)(x >> 32));
+}
+
+__device__ __forceinline__ void gst(uint2 *p, uint64_t i, uint2 v)
+{
+    reinterpret_cast<gu64_t *>(reinterpret_cast<uintptr_t>(p))[i] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+}
+
+__device__ __forceinline__ uint32_t gld32(const uint32_t *p, uint64_t i)
+{
+    return reinterpret_cast<const gu32_t *>(reinterpret_cast<uintptr_t>(p))[i];
+}
+
+__device__ __forceinline__ void gst32(uint32_t *p, uint64_t i, uint32_t v)
+{
+    reinterpret_cast<gu32_t *>(reinterpret_cast<uintptr_t>(p))[i] = v;
+}
+
+// ---------------------------------------------------------------------------
 // Hashes (device).  Same functions as the reference's src/dpu/src/hash.c.
 // ---------------------------------------------------------------------------
 
@@ -122,6 +152,16 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
     return x * q + min(x, r) + j;
 }
 
+// Number of heavy keys of an instance, read through the scalar (constant)
+// cache: the count is written by an earlier kernel and only read here, and a
+// scalar load does not wait behind the vector loads and stores in flight
+// (a vector load would, on the shared vmcnt counter).
+__device__ __forceinline__ uint32_t heavy_keys_of(const InstDesc &I)
+{
+    typedef const __attribute__((address_space(4))) uint32_t const_u32;
+    return *reinterpret_cast<const_u32 *>(reinterpret_cast<uintptr_t>(I.heavy_count));
+}
+
 // Load the instance's heavy table into LDS and build its membership bitmap
 // (kHeavyFilterBits bits; a key whose bit is clear is not heavy).
 __device__ __forceinline__ uint32_t heavy_bit(uint32_t key) { return (key * 0xC2B2AE35u) >> (32 - kHeavyFilterLog2); }
@@ -131,9 +171,9 @@ __device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint
     for (uint32_t i = threadIdx.x; i < (1u << kHeavyFilterLog2) / 32; i += blockDim.x) hf[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        const uint32_t k = I.heavy_keys[i];
+        const uint32_t k = gld32(I.heavy_keys, i);
         hk[i] = k;
-        hr[i] = I.heavy_rids[i];
+        hr[i] = gld32(I.heavy_rids, i);
         if (k != kEmptyKey) {
             const uint32_t b = heavy_bit(k);
             atomicOr(&hf[b >> 5], 1u << (b & 31u));
@@ -188,7 +228,7 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
     uint32_t *hr = hk + kHeavySlots;      // kHeavySlots
     uint32_t *hf = hr + kHeavySlots;      // heavy bitmap
     uint32_t *hist = hf + (1u << kHeavyFilterLog2) / 32;   // F
-    const bool heavy = I.heavy != 0 && *I.heavy_count != 0;
+    const bool heavy = I.heavy != 0 && heavy_keys_of(I) != 0;
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
     if (heavy) load_heavy(I, hk, hr, hf);
     __syncthreads();
@@ -200,7 +240,7 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
 #pragma unroll
         for (uint32_t j = 0; j < U; j++) {
             const uint32_t i = b + j * blockDim.x + threadIdx.x;
-            v[j] = (i < td.end) ? in[i] : make_uint2(kEmptyKey, 0u);
+            v[j] = (i < td.end) ? gld(in, i) : make_uint2(kEmptyKey, 0u);
         }
         // heavy lookups of all U tuples first, so their LDS reads overlap
         int hs[U];
@@ -211,7 +251,7 @@ __global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts
             if (v[j].x == kEmptyKey) continue;
             if (mode == 2 && hs[j] >= 0) continue;   // heavy S tuple: no partition
             atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
-            if (mode == 1 && hs[j] >= 0) I.heavy_rids[hs[j]] = v[j].y;   // R keys are unique: one writer per slot
+            if (mode == 1 && hs[j] >= 0) gst32(I.heavy_rids, hs[j], v[j].y);   // R keys are unique: one writer per slot
         }
     }
     __syncthreads();
@@ -333,7 +373,26 @@ __device__ __forceinline__ void load_tile_round(uint2 *v, const InstDesc *__rest
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         const uint32_t li = j * NT + threadIdx.x;
-        if (li < rn) v[j] = in[o + li];
+        if (li < rn) v[j] = gld(in, o + li);
+    }
+}
+
+// Write out a staged round: partition tuples stage[0, hstart) go to
+// outp[bs[digit] + i] (the digit is recomputed from the key, which is
+// cheaper than staging it), heavy pairs stage[hstart, total) -- whose key
+// was replaced by the R rid -- go to out[hb + i - hstart].
+__device__ __forceinline__ void scatter_write_out(const uint2 *stage, const uint32_t *bs, uint32_t mask,
+                                                  uint32_t hstart, uint32_t total, uint2 *outp, uint2 *out,
+                                                  uint32_t hb, uint32_t nS, uint32_t *err_inst)
+{
+    for (uint32_t i = threadIdx.x; i < hstart; i += kScatterNT) {
+        const uint2 t = stage[i];
+        gst(outp, bs[sfh(t.x) & mask] + i, t);
+    }
+    for (uint32_t i = hstart + threadIdx.x; i < total; i += kScatterNT) {
+        const uint32_t oi = hb + (i - hstart);
+        if (oi < nS) gst(out, oi, stage[i]);
+        else atomicOr(err_inst, kErrOutOverflow);
     }
 }
 
@@ -389,7 +448,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 
         const InstDesc &I = insts[td.inst];
         const uint32_t F = 1u << I.bits, mask = F - 1u;
-        const bool heavy = (I.heavy != 0) && (td.rel == 1) && *I.heavy_count != 0;
+        const bool heavy = (I.heavy != 0) && (td.rel == 1) && heavy_keys_of(I) != 0;
         const uint32_t rn = min(ROUND, td.end - o);
         __syncthreads();   // the previous round's write-out and clearing are done
         if (heavy && heavy_inst != td.inst) {
@@ -484,21 +543,8 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             misc[0] = hn ? I.nS - hres - hn : 0u;
         }
         __syncthreads();
-        uint2 *outp = td.rel ? I.Sp : I.Rp;
-        // a staged tuple's digit is recomputed from its key (cheaper than
-        // staging it); heavy tuples, whose key was replaced, lie beyond hstart
-        for (uint32_t i = threadIdx.x; i < hstart; i += NT) {
-            const uint2 t = stage[i];
-            outp[bs[sfh(t.x) & mask] + i] = t;
-        }
-        if (heavy) {
-            const uint32_t hb = misc[0];
-            for (uint32_t i = hstart + threadIdx.x; i < total; i += NT) {
-                const uint32_t oi = hb + (i - hstart);
-                if (oi < I.nS) I.out[oi] = stage[i];   // .x already holds the R rid
-                else atomicOr(&err[td.inst], kErrOutOverflow);
-            }
-        }
+        scatter_write_out(stage, bs, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
+                          &err[td.inst]);
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         if (!more) break;
         t = tn;
@@ -543,7 +589,7 @@ __global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict
     const uint64_t b = (uint64_t)(blockIdx.x - job.first_block) * job.stride * kSampleBlock;
     const uint32_t e = (uint32_t)min<uint64_t>(I.nS, b + kSampleBlock);
     for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        const uint32_t key = I.S[i].x;
+        const uint32_t key = gld(I.S, i).x;
         if (key == kEmptyKey) continue;
         uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
         for (uint32_t step = 0; step < kSampleSlots; step++) {
@@ -608,8 +654,8 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        I.heavy_keys[i] = tab[i];
-        I.heavy_rids[i] = kNoRid;
+        gst32(I.heavy_keys, i, tab[i]);
+        gst32(I.heavy_rids, i, kNoRid);
     }
     if (threadIdx.x == 0) nheavy[job.inst] = placed;
 }
@@ -703,7 +749,7 @@ __device__ __forceinline__ void wave_emit(bool hit, uint32_t r_rid, uint32_t s_r
     wb = __shfl(wb, 0, kWave);
     if (hit) {
         const uint32_t pos = wb + mbcnt(m);
-        if (pos < sh->res) out[sh->base + pos] = make_uint2(r_rid, s_rid);
+        if (pos < sh->res) gst(out, sh->base + pos, make_uint2(r_rid, s_rid));
         else atomicOr(err_i, kErrOutOverflow);
     }
 }
@@ -730,7 +776,7 @@ __device__ __forceinline__ void wave_emit_round(uint32_t hitm, const uint2 *p, J
     for (int j = 0; j < N; j++) {
         if ((hitm >> j) & 1u) {
             const uint32_t pos = wb + mbcnt(m[j]);
-            if (pos < res) out[base + pos] = p[j];
+            if (pos < res) gst(out, base + pos, p[j]);
             else atomicOr(err_i, kErrOutOverflow);
         }
         wb += (uint32_t)__popcll(m[j]);
@@ -761,7 +807,7 @@ __device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, uint32_t s0
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         const uint32_t si = s0 + j * kJoinNT + threadIdx.x;
-        v[j] = (si < se) ? S[si] : make_uint2(kEmptyKey, 0u);
+        v[j] = (si < se) ? gld(S, si) : make_uint2(kEmptyKey, 0u);
     }
 }
 
@@ -803,7 +849,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             for (uint32_t c0 = 0; c0 < nr; c0 += kNLChunk) {
                 const uint32_t cn = min(kNLChunk, nr - c0);
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = I.Rp[rb + c0 + i]; rk[i] = r.x; rv[i] = r.y; }
+                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = gld(I.Rp, rb + c0 + i); rk[i] = r.x; rv[i] = r.y; }
                 __syncthreads();
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
@@ -838,7 +884,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 while (np < cn) np <<= 1;
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < np; i += NT) {
-                    if (i < cn) { uint2 r = I.Rp[rb + c0 + i]; ra[i] = ((uint64_t)r.x << 32) | r.y; }
+                    if (i < cn) { uint2 r = gld(I.Rp, rb + c0 + i); ra[i] = ((uint64_t)r.x << 32) | r.y; }
                     else ra[i] = ~0ull;
                 }
                 __syncthreads();
@@ -848,7 +894,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     uint32_t sp = 1;
                     while (sp < sn) sp <<= 1;
                     for (uint32_t i = threadIdx.x; i < sp; i += NT) {
-                        if (i < sn) { uint2 sv = I.Sp[s0 + i]; sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
+                        if (i < sn) { uint2 sv = gld(I.Sp, s0 + i); sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
                         else sa[i] = ~0ull;
                     }
                     __syncthreads();
@@ -893,7 +939,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         if (takeA) {
                             const uint32_t k = (uint32_t)(sa[ai] >> 32);
                             if (k != kEmptyKey && bi < cn && (uint32_t)(ra[bi] >> 32) == k) {
-                                if (o < sh.res) I.out[sh.base + o] = make_uint2((uint32_t)ra[bi], (uint32_t)sa[ai]);
+                                if (o < sh.res) gst(I.out, sh.base + o, make_uint2((uint32_t)ra[bi], (uint32_t)sa[ai]));
                                 else atomicOr(erri, kErrOutOverflow);
                                 o++;
                             }
@@ -934,7 +980,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = rb + u * NT + threadIdx.x;
-                        if (i < re) r[u] = I.Rp[i];
+                        if (i < re) r[u] = gld(I.Rp, i);
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++)
@@ -945,7 +991,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                            c[u] = (i < re) ? gld(I.Rp, i) : make_uint2(kEmptyKey, 0u);
                         }
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
@@ -979,7 +1025,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? I.Rp[i] : make_uint2(kEmptyKey, 0u);
+                            c[u] = (i < re) ? gld(I.Rp, i) : make_uint2(kEmptyKey, 0u);
                         }
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
