@@ -430,6 +430,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     // are loaded while this round is ranked, staged and written
     uint2 nv[IPT];
     load_tile_round<IPT>(nv, insts, td, o);
+    __builtin_amdgcn_s_waitcnt(0);   // entering the loop with no loads in flight (see the wait below)
     for (;;) {
         uint32_t tn = t, on = o + ROUND;
         TileDesc tdn = td;
@@ -532,6 +533,12 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 stage[p] = v[j];
             }
         }
+        // One unconditional wait for all vector memory operations: the
+        // reservations, and with them (completion is in issue order) the
+        // next round's loads.  The compiler then knows those loads are done
+        // at the next round's start and does not wait there -- a wait that
+        // would also cover this round's stores, issued after them.
+        __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
             const uint32_t d = u * NT + threadIdx.x;
@@ -1051,6 +1058,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
                 load_round<IPT>(nv, I.Sp, w.sb, w.se);
+                __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
 #pragma unroll
@@ -1081,6 +1089,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
                         hitm |= (hit ? 1u : 0u) << j;
                     }
+                    // wait here for the next round's loads (issued a probe
+                    // round ago), not at the next round's start, where the
+                    // wait would also cover the stores below
+                    __builtin_amdgcn_s_waitcnt(0);
                     wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
                 }
             }
