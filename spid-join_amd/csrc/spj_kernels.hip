@@ -829,12 +829,15 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
     __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
     constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
     const uint32_t total_items = ctrl[0];
+    if (threadIdx.x == 0) sh.item = atomicAdd(&ctrl[1], 1u);
     for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) sh.item = atomicAdd(&ctrl[1], 1u);
         __syncthreads();
         const uint32_t it = sh.item;
         if (it >= total_items) break;
+        // the next item's index is taken now and published at the end of
+        // this item, so the queue's round trip hides behind this item
+        uint32_t next_item = 0;
+        if (threadIdx.x == 0) next_item = atomicAdd(&ctrl[1], 1u);
         const WorkItem w = items[it];
         const InstDesc &I = insts[w.inst];
         const uint32_t rb = pstart[I.pstartR + w.part], re = pstart[I.pstartR + w.part + 1];
@@ -842,10 +845,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
         uint32_t *erri = err + w.inst;
+        // the item's output reservation; the hash path publishes the base
+        // after its table build, which hides the atomic's round trip
+        uint32_t res_base = 0;
         if (threadIdx.x == 0) {
             sh.res = w.se - w.sb;
-            sh.base = atomicAdd(&out_lo[w.inst], sh.res);
             sh.fill = 0;
+            res_base = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+            if (alg != 0) sh.base = res_base;
         }
         __syncthreads();
 
@@ -1009,6 +1016,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 __syncthreads();
                 const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                if (q == 0 && threadIdx.x == 0) sh.base = res_base;   // read after the barriers below
                 if (ntab > kTableCap) {
                     if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
                     continue;   // uniform: ntab is the same in every thread
@@ -1045,13 +1053,28 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 __syncthreads();   // bend[b] = end of bucket b
                 // duplicate R keys (a2a_srj_build.c:265-271): each entry against
-                // the later entries of its bucket
+                // the later entries of its bucket, in phases (keys, bucket ends,
+                // next entry) so that the LDS reads of all entries overlap;
+                // buckets of more than two entries finish in a short loop
                 {
                     bool dup = false;
-                    for (uint32_t i = threadIdx.x; i < ntab; i += NT) {
-                        const uint32_t k = tkv[i].x;
-                        const uint32_t en = bend[bucket_of(k, lb)];
-                        for (uint32_t j = i + 1; j < en; j++) dup |= (tkv[j].x == k);
+                    uint32_t kk[RPT], ee[RPT];
+#pragma unroll
+                    for (uint32_t u = 0; u < RPT; u++) {
+                        const uint32_t i = u * NT + threadIdx.x;
+                        kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb)] : 0u;
+#pragma unroll
+                    for (uint32_t u = 0; u < RPT; u++) {
+                        const uint32_t i = u * NT + threadIdx.x;
+                        if (i + 1 < ee[u]) dup |= (tkv[i + 1].x == kk[u]);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < RPT; u++) {
+                        const uint32_t i = u * NT + threadIdx.x;
+                        for (uint32_t j = i + 2; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
                     }
                     if (dup) atomicOr(erri, kErrDuplicateKey);
                 }
@@ -1105,6 +1128,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 const uint32_t h = atomicAdd(&ctrl[2], 1u);
                 if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
             }
+            sh.item = next_item;
         }
     }
 }
