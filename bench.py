@@ -106,12 +106,14 @@ def measured_traffic(kernel):
     plain-copy calibration) from the committed rocprofv3 counter summary of
     this same workload (profiles/c2_traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "c2_traffic.json")
-    key = {"k_scatter": "k_scatter<16>"}.get(kernel, kernel)
     try:
         with open(path) as f:
-            t = json.load(f)["per_dispatch"][key]
+            per = json.load(f)["per_dispatch"]
+        # template instances carry their argument (k_scatter<12>)
+        key = next(k for k in per if k == kernel or k.startswith(kernel + "<"))
+        t = per[key]
         return int(t["fetch_bytes"] + t["write_bytes"])
-    except (OSError, KeyError, ValueError):
+    except (OSError, KeyError, ValueError, StopIteration):
         return None
 
 

@@ -84,18 +84,19 @@ struct ScanBlock {
 
 constexpr uint32_t kScanTileChunk = 32;   // tiles per column-scan chunk
 
-// Heavy-key detection of one instance: its sample blocks are
-// first_block .. first_block + nblocks - 1 of the sampling grid, every
-// stride-th 8K-tuple block of S; its candidate map lives at cand_off.
+// Heavy-key detection of one instance: nblocks sample blocks, every
+// stride-th 8K-tuple block of S.  Phase 0 samples every kPreSample-th of
+// them; phase 1 samples the rest, and only if phase 0 found a candidate.
+// first_block[p] is the job's first block in phase p's launch.
 struct HeavyJob {
     uint32_t inst;
     uint32_t stride;
-    uint32_t first_block;
+    uint32_t first_block[2];
     uint32_t nblocks;
     uint32_t min_count;
-    uint32_t pad;
-    uint64_t cand_off;
+    uint64_t cand_off;       // candidate map: keys, counts, then the found flag
 };
+constexpr uint32_t kPreSample = 16;
 
 // One unit of join work: partition p of instance inst, S range [sb, se)
 // (absolute positions in the instance's partitioned S region).

@@ -234,7 +234,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
     if (nheavy_inst) {
         SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
-        SPJ_TRY(cand_.ensure((size_t)nheavy_inst * 2 * cand_slots() * 4));
+        SPJ_TRY(cand_.ensure((size_t)nheavy_inst * cand_words() * 4));
     }
     for (size_t i = 0; i < n; i++) {
         InstDesc &d = h_insts_[i];
@@ -268,25 +268,29 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
     SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
 
-    // ---- heavy keys: one sampling launch and one selection launch for all
+    // ---- heavy keys: two sampling launches (pre-sample, then the rest when
+    // the pre-sample saw a candidate) and one selection launch for all
     if (nheavy_inst) {
         h_jobs_.clear();
-        uint32_t blocks = 0;
+        uint32_t blocks[2] = {0, 0};
         for (size_t i = 0; i < n; i++) {
             if (heavy_idx[i] < 0) continue;
             HeavyJob j{};
             j.inst = (uint32_t)i;
-            j.first_block = blocks;
             j.nblocks = heavy_sample_blocks(specs[i].nS, 1024, &j.stride);
+            const uint32_t pre = (j.nblocks + kPreSample - 1) / kPreSample;
+            j.first_block[0] = blocks[0];
+            j.first_block[1] = blocks[1];
             j.min_count = specs[i].cfg.heavy_min;
-            j.cand_off = (uint64_t)heavy_idx[i] * 2 * cand_slots();
-            blocks += j.nblocks;
+            j.cand_off = (uint64_t)heavy_idx[i] * cand_words();
+            blocks[0] += pre;
+            blocks[1] += j.nblocks - pre;
             h_jobs_.push_back(j);
         }
         SPJ_TRY(jobs_.ensure(h_jobs_.size() * sizeof(HeavyJob)));
         SPJ_TRY(hipMemcpyAsync(jobs_.p, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob), hipMemcpyHostToDevice, st));
         SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
-        SPJ_TRY(launch_fill(cand_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * cand_slots(), 0u, st));
+        SPJ_TRY(launch_fill(cand_.as<uint32_t>(), (uint64_t)nheavy_inst * cand_words(), 0u, st));
         SPJ_TRY(launch_heavy_detect(d_insts, jobs_.as<HeavyJob>(), (uint32_t)h_jobs_.size(), blocks,
                                     cand_.as<uint32_t>(), nheavy, st));
     }

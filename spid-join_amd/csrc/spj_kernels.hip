@@ -572,31 +572,49 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 // position -- the same multiset as the plain join.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSampleBlock = 8192;
+constexpr uint32_t kGateMin = 8;          // pre-sample count of one key that opens the full sample
 constexpr uint32_t kSampleSlots = 16384;
 constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
 
 __global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
-                                                      uint32_t njobs, uint32_t *__restrict__ cand_base)
+                                                      uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase)
 {
     extern __shared__ uint32_t lds[];
     uint32_t *sk = lds;
     uint32_t *sc = lds + kSampleSlots;
-    // this block's job: last job whose first_block <= blockIdx.x
+    // this block's job: last job whose first_block[phase] <= blockIdx.x
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t m = (lo + hi) >> 1;
-        if (jobs[m].first_block <= blockIdx.x) lo = m; else hi = m;
+        if (jobs[m].first_block[phase] <= blockIdx.x) lo = m; else hi = m;
     }
     const HeavyJob job = jobs[lo];
     const InstDesc &I = insts[job.inst];
     uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
+    uint32_t *found = cand_cnt + kCandSlots;
+    // sample block of this workgroup: phase 0 takes blocks 0, kPreSample,
+    // 2 kPreSample, ...; phase 1 the others
+    const uint32_t l = blockIdx.x - job.first_block[phase];
+    const uint32_t k = phase == 0 ? l * kPreSample : (l / (kPreSample - 1)) * kPreSample + l % (kPreSample - 1) + 1;
+    if (k >= job.nblocks) return;
+    if (phase == 1 && *found == 0) return;   // no candidate in the pre-sample: no skew worth the rest
     for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
     __syncthreads();
-    const uint64_t b = (uint64_t)(blockIdx.x - job.first_block) * job.stride * kSampleBlock;
-    const uint32_t e = (uint32_t)min<uint64_t>(I.nS, b + kSampleBlock);
-    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        const uint32_t key = gld(I.S, i).x;
+    const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
+    const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
+    // all of the block's keys are loaded first (kSampleBlock / 512 per
+    // thread), so the loads overlap instead of waiting one by one
+    constexpr uint32_t KPT = kSampleBlock / 512;
+    uint32_t keys[KPT];
+#pragma unroll
+    for (uint32_t u = 0; u < KPT; u++) {
+        const uint64_t i = b + u * 512 + threadIdx.x;
+        keys[u] = (i < e) ? gld(I.S, i).x : kEmptyKey;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < KPT; u++) {
+        const uint32_t key = keys[u];
         if (key == kEmptyKey) continue;
         uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
         for (uint32_t step = 0; step < kSampleSlots; step++) {
@@ -612,7 +630,13 @@ __global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict
         uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
         for (uint32_t step = 0; step < kCandSlots; step++) {
             uint32_t old = atomicCAS(&cand_keys[s], kEmptyKey, key);
-            if (old == kEmptyKey || old == key) { atomicAdd(&cand_cnt[s], sc[i]); break; }
+            if (old == kEmptyKey || old == key) {
+                const uint32_t sum = atomicAdd(&cand_cnt[s], sc[i]) + sc[i];
+                // open the second phase only on a key seen often across the
+                // pre-sample, not on one chance cluster in a single block
+                if (phase == 0 && sum >= kGateMin) *found = 1u;
+                break;
+            }
             s = (s + 1u) & (kCandSlots - 1u);
         }
     }
@@ -1285,19 +1309,20 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
     return total ? (total + st - 1) / st : 0;
 }
 
-hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, uint32_t total_blocks,
+hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
                                uint32_t *cand_base, uint32_t *nheavy, hipStream_t st)
 {
-    if (!njobs || !total_blocks) return hipSuccess;
+    if (!njobs || !blocks[0]) return hipSuccess;
     size_t l = 2 * kSampleSlots * 4;
     hipError_t e = set_lds((const void *)k_heavy_sample, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_heavy_sample, dim3(total_blocks), dim3(512), l, st, insts, jobs, njobs, cand_base);
+    hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[0]), dim3(512), l, st, insts, jobs, njobs, cand_base, 0u);
+    if (blocks[1]) hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[1]), dim3(512), l, st, insts, jobs, njobs, cand_base, 1u);
     hipLaunchKernelGGL(k_heavy_select, dim3(njobs), dim3(1024), 0, st, insts, jobs, cand_base, nheavy);
     return hipGetLastError();
 }
 
-uint32_t cand_slots() { return kCandSlots; }
+uint32_t cand_words() { return 2 * kCandSlots + 4; }
 
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
                              WorkItem *items, uint32_t *nitems, hipStream_t st)

@@ -12,7 +12,7 @@ size_t lds_hist(uint32_t max_bits);
 size_t lds_scatter(uint32_t max_bits, int ipt);
 size_t lds_join();
 uint32_t join_table_cap();
-uint32_t cand_slots();
+uint32_t cand_words();   // words of one job's candidate map (keys, counts, found flag)
 
 hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
@@ -27,10 +27,12 @@ hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t
 // Sample blocks for heavy detection of an S of nS tuples (at most
 // max_blocks, evenly strided); *stride receives the stride in blocks.
 uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride);
-// Heavy-key detection for every job at once: one sampling launch over all
-// jobs' sample blocks, one selection workgroup per job.  Each job owns
-// 2 * cand_slots() words of zeroed candidate map at cand_base + cand_off.
-hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, uint32_t total_blocks,
+// Heavy-key detection for every job at once: a sampling launch over every
+// job's pre-sample blocks (blocks[0] in all), a second one over the other
+// sample blocks (blocks[1]; a job skips them when its pre-sample found no
+// candidate), then one selection workgroup per job.  Each job owns
+// cand_words() zeroed words at cand_base + cand_off.
+hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
                                uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
                              WorkItem *items, uint32_t *nitems, hipStream_t st);
