@@ -54,16 +54,17 @@ def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
             n, _t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=False)
             if n != nS:
                 raise RuntimeError(f"join produced {n} pairs, expected {nS}")
-        phases = []
         sj.check(sj.lib().spj_dev_sync())
         t1 = time.perf_counter()
         for _ in range(steps):
-            n, t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)
-            phases.append(t)
+            n, _t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=False)
         sj.check(sj.lib().spj_dev_sync())
         wall = time.perf_counter() - t1
         if n != nS:
             raise RuntimeError(f"join produced {n} pairs, expected {nS}")
+        # per-phase breakdown: HIP events between the phases, in extra joins
+        # after the timed region (the events themselves cost ~2% of a step)
+        phases = [sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)[1] for _ in range(3)]
     finally:
         dS.free()
         dO.free()

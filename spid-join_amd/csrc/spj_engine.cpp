@@ -21,6 +21,33 @@ namespace spj {
         if (_e != hipSuccess) return _e;            \
     } while (0)
 
+hipError_t PinBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = std::max<size_t>(bytes, 4096);
+    want = (want + 65535) & ~size_t(65535);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void PinBuf::release()
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 hipError_t DevBuf::ensure(size_t bytes)
 {
     if (bytes <= cap && p) return hipSuccess;
@@ -103,6 +130,7 @@ Context::Context()
 {
     (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     for (auto &e : ev_) (void)hipEventCreate(&e);
+    (void)hipEventCreateWithFlags(&desc_ev_, hipEventDisableTiming);
 }
 
 Context::~Context()
@@ -110,9 +138,12 @@ Context::~Context()
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
-    DevBuf *bufs[] = {&insts_, &tiles_, &counts_, &part_, &pstart_, &segs_, &blk_ad_, &blk_b_, &rp_,
-                      &sp_, &items_, &ctrl_, &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &jobs_};
+    if (desc_ev_) (void)hipEventDestroy(desc_ev_);
+    DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
+                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_};
     for (auto *b : bufs) b->release();
+    pin_desc_.release();
+    pin_ctrl_.release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -217,15 +248,38 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         }
     }
 
+    // ---- heavy-key jobs (sampling blocks of both phases)
+    h_jobs_.clear();
+    uint32_t hblocks[2] = {0, 0};
+    for (size_t i = 0; i < n; i++) {
+        if (heavy_idx[i] < 0) continue;
+        HeavyJob j{};
+        j.inst = (uint32_t)i;
+        j.nblocks = heavy_sample_blocks(specs[i].nS, 1024, &j.stride);
+        const uint32_t pre = (j.nblocks + kPreSample - 1) / kPreSample;
+        j.first_block[0] = hblocks[0];
+        j.first_block[1] = hblocks[1];
+        j.min_count = specs[i].cfg.heavy_min;
+        j.cand_off = (uint64_t)heavy_idx[i] * cand_words();
+        hblocks[0] += pre;
+        hblocks[1] += j.nblocks - pre;
+        h_jobs_.push_back(j);
+    }
+
     // ---- buffers
-    SPJ_TRY(insts_.ensure(n * sizeof(InstDesc)));
-    SPJ_TRY(tiles_.ensure(h_tiles_.size() * sizeof(TileDesc)));
+    // descriptor block layout (256-byte aligned arrays)
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_insts = 0;
+    const size_t o_tiles = o_insts + al(n * sizeof(InstDesc));
+    const size_t o_segs = o_tiles + al(h_tiles_.size() * sizeof(TileDesc));
+    const size_t o_ad = o_segs + al(h_segs_.size() * sizeof(ScanSeg));
+    const size_t o_b = o_ad + al(h_ad_.size() * sizeof(ScanBlock));
+    const size_t o_jobs = o_b + al(h_b_.size() * sizeof(ScanBlock));
+    const size_t desc_bytes = o_jobs + al(h_jobs_.size() * sizeof(HeavyJob));
+    SPJ_TRY(desc_.ensure(desc_bytes));
     SPJ_TRY(counts_.ensure(cnt_total * 4));
     SPJ_TRY(part_.ensure(part_total * 4));
     SPJ_TRY(pstart_.ensure(pst_total * 4));
-    SPJ_TRY(segs_.ensure(h_segs_.size() * sizeof(ScanSeg)));
-    SPJ_TRY(blk_ad_.ensure(h_ad_.size() * sizeof(ScanBlock)));
-    SPJ_TRY(blk_b_.ensure(h_b_.size() * sizeof(ScanBlock)));
     SPJ_TRY(rp_.ensure(rp_total * 8));
     SPJ_TRY(sp_.ensure(sp_total * 8));
     SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
@@ -248,19 +302,34 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     }
 
     hipStream_t st = stream_;
-    SPJ_TRY(hipMemcpyAsync(insts_.p, h_insts_.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, st));
-    if (!h_tiles_.empty())
-        SPJ_TRY(hipMemcpyAsync(tiles_.p, h_tiles_.data(), h_tiles_.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
-    SPJ_TRY(hipMemcpyAsync(segs_.p, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg), hipMemcpyHostToDevice, st));
-    if (!h_ad_.empty())
-        SPJ_TRY(hipMemcpyAsync(blk_ad_.p, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
-    SPJ_TRY(hipMemcpyAsync(blk_b_.p, h_b_.data(), h_b_.size() * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
+    // one upload of all descriptors through page-locked staging; the
+    // staging is reused once the previous upload has left it
+    if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));
+    SPJ_TRY(pin_desc_.ensure(desc_bytes));
+    {
+        char *h = pin_desc_.as<char>();
+        if (n) memcpy(h + o_insts, h_insts_.data(), n * sizeof(InstDesc));
+        if (!h_tiles_.empty()) memcpy(h + o_tiles, h_tiles_.data(), h_tiles_.size() * sizeof(TileDesc));
+        if (!h_segs_.empty()) memcpy(h + o_segs, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg));
+        if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
+        if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
+        if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
+    }
+    SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
+    SPJ_TRY(hipEventRecord(desc_ev_, st));
+    desc_pending_ = true;
+    char *dd = desc_.as<char>();
+    const TileDesc *d_tiles = reinterpret_cast<const TileDesc *>(dd + o_tiles);
+    const ScanSeg *d_segs = reinterpret_cast<const ScanSeg *>(dd + o_segs);
+    const ScanBlock *d_ad = reinterpret_cast<const ScanBlock *>(dd + o_ad);
+    const ScanBlock *d_b = reinterpret_cast<const ScanBlock *>(dd + o_b);
+    const HeavyJob *d_jobs = reinterpret_cast<const HeavyJob *>(dd + o_jobs);
     for (size_t i = 0; i < n; i++)   // resident tables: their partition starts
         if (specs[i].Rp_pre)
             SPJ_TRY(hipMemcpyAsync(pstart_.as<uint32_t>() + h_insts_[i].pstartR, specs[i].pst_pre,
                                    ((size_t(1) << specs[i].cfg.bits) + 1) * 4, hipMemcpyDeviceToDevice, st));
 
-    const InstDesc *d_insts = insts_.as<InstDesc>();
+    const InstDesc *d_insts = reinterpret_cast<const InstDesc *>(dd + o_insts);
     uint32_t *ctrl = ctrl_.as<uint32_t>();
     uint32_t *out_lo = ctrl + ctrl_at(n, kCtrlLo, 0), *out_hi = ctrl + ctrl_at(n, kCtrlHi, 0);
     uint32_t *out_fill = ctrl + ctrl_at(n, kCtrlFill, 0), *err = ctrl + ctrl_at(n, kCtrlErr, 0);
@@ -271,42 +340,24 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     // ---- heavy keys: two sampling launches (pre-sample, then the rest when
     // the pre-sample saw a candidate) and one selection launch for all
     if (nheavy_inst) {
-        h_jobs_.clear();
-        uint32_t blocks[2] = {0, 0};
-        for (size_t i = 0; i < n; i++) {
-            if (heavy_idx[i] < 0) continue;
-            HeavyJob j{};
-            j.inst = (uint32_t)i;
-            j.nblocks = heavy_sample_blocks(specs[i].nS, 1024, &j.stride);
-            const uint32_t pre = (j.nblocks + kPreSample - 1) / kPreSample;
-            j.first_block[0] = blocks[0];
-            j.first_block[1] = blocks[1];
-            j.min_count = specs[i].cfg.heavy_min;
-            j.cand_off = (uint64_t)heavy_idx[i] * cand_words();
-            blocks[0] += pre;
-            blocks[1] += j.nblocks - pre;
-            h_jobs_.push_back(j);
-        }
-        SPJ_TRY(jobs_.ensure(h_jobs_.size() * sizeof(HeavyJob)));
-        SPJ_TRY(hipMemcpyAsync(jobs_.p, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob), hipMemcpyHostToDevice, st));
         SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
         SPJ_TRY(launch_fill(cand_.as<uint32_t>(), (uint64_t)nheavy_inst * cand_words(), 0u, st));
-        SPJ_TRY(launch_heavy_detect(d_insts, jobs_.as<HeavyJob>(), (uint32_t)h_jobs_.size(), blocks,
-                                    cand_.as<uint32_t>(), nheavy, st));
+        SPJ_TRY(launch_heavy_detect(d_insts, d_jobs, (uint32_t)h_jobs_.size(), hblocks, cand_.as<uint32_t>(), nheavy,
+                                    st));
     }
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 
     // ---- partition: histogram, scan, scatter
     const uint32_t ntiles = (uint32_t)h_tiles_.size();
     SPJ_TRY(launch_fill(counts_.as<uint32_t>(), cnt_total, 0u, st));
-    SPJ_TRY(launch_hist(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), max_bits_, st));
+    SPJ_TRY(launch_hist(d_insts, d_tiles, ntiles, counts_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
-    SPJ_TRY(launch_scan(segs_.as<ScanSeg>(), (uint32_t)h_segs_.size(), blk_ad_.as<ScanBlock>(), (uint32_t)h_ad_.size(),
-                        blk_b_.as<ScanBlock>(), (uint32_t)h_b_.size(), counts_.as<uint32_t>(), part_.as<uint32_t>(),
+    SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), d_ad, (uint32_t)h_ad_.size(), d_b, (uint32_t)h_b_.size(),
+                        counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
     // one persistent 1024-thread workgroup per CU (LDS-bound)
-    SPJ_TRY(launch_scatter(d_insts, tiles_.as<TileDesc>(), ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
+    SPJ_TRY(launch_scatter(d_insts, d_tiles, ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
                            max_ipt_, (uint32_t)std::max(1, device_info().cus), st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
@@ -360,8 +411,11 @@ hipError_t Context::compact(size_t i, uint32_t lo, uint32_t hi, uint32_t fill, c
 hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed)
 {
     h_ctrl_.resize(ctrl_words(n));
-    SPJ_TRY(hipMemcpyAsync(h_ctrl_.data(), ctrl_.p, ctrl_words(n) * 4, hipMemcpyDeviceToHost, stream_));
+    SPJ_TRY(pin_ctrl_.ensure(ctrl_words(n) * 4));
+    SPJ_TRY(hipMemcpyAsync(pin_ctrl_.p, ctrl_.p, ctrl_words(n) * 4, hipMemcpyDeviceToHost, stream_));
     SPJ_TRY(hipStreamSynchronize(stream_));
+    desc_pending_ = false;
+    memcpy(h_ctrl_.data(), pin_ctrl_.p, ctrl_words(n) * 4);
     counts.resize(n);
     errs.assign(h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0), h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0) + n);
     std::vector<uint4> holes;

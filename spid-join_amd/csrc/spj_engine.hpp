@@ -53,6 +53,17 @@ struct DevBuf {
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
+// Page-locked host memory (hipHostMalloc), grown on demand: staging for the
+// descriptor upload and the control-word read-back, so both copies are
+// asynchronous DMA instead of staged pageable copies.
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
 struct DeviceInfo {
     int device = 0;
     int cus = 0;
@@ -90,8 +101,13 @@ public:
 private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
-    DevBuf insts_, tiles_, counts_, part_, pstart_, segs_, blk_ad_, blk_b_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, jobs_;
+    // desc_: every descriptor array of a batch (instances, tiles, scan
+    // segments and blocks, heavy jobs) in one block, uploaded by one copy
+    DevBuf desc_, counts_, part_, pstart_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_;
+    PinBuf pin_desc_, pin_ctrl_;
+    hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
+    bool desc_pending_ = false;
     std::vector<HeavyJob> h_jobs_;
     uint32_t max_holes_ = 0;
     size_t specs_n_ = 0;
