@@ -993,13 +993,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             }
         } else {
             // ---------------- hash join
-            // LDS: (key, rid) pairs counting-sorted by bucket, then one 16-bit
-            // end offset per bucket: bucket b is [bend[b - 1], bend[b]).
+            // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
+            // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
+            // bucket b is [bend[b], bend[b + 1]).
             // During the count pass the 32-bit bucket counters live in the
             // pair area, which the fill pass overwrites afterwards.
             uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
             uint32_t *bcnt = lds;                           // nb counters (count pass only)
-            uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets
+            uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
             uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
             const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
             const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
@@ -1045,8 +1046,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
                     continue;   // uniform: ntab is the same in every thread
                 }
-                // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap)
-                for (uint32_t i = threadIdx.x; i < nb; i += NT) bend[i] = (uint16_t)bcnt[i];
+                // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
+                // cursor of bucket b is bend[b + 1]
+                for (uint32_t i = threadIdx.x; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                if (threadIdx.x == 0) bend[0] = 0;
                 __syncthreads();
                 // fill pass: a packed atomic advances one 16-bit cursor of a
                 // word; it never carries, every cursor staying <= ntab
@@ -1054,8 +1057,8 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         if (r[u].x == kEmptyKey) continue;
-                        const uint32_t bk = bucket_of(r[u].x, lb), sh16 = (bk & 1u) * 16u;
-                        const uint32_t pos = (atomicAdd(&bend_w[bk >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                        const uint32_t c = bucket_of(r[u].x, lb) + 1u, sh16 = (c & 1u) * 16u;
+                        const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
                         tkv[pos] = r[u];
                     }
                 } else {
@@ -1069,13 +1072,13 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                            const uint32_t bk = bucket_of(c[u].x, lb), sh16 = (bk & 1u) * 16u;
-                            const uint32_t pos = (atomicAdd(&bend_w[bk >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                            const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
+                            const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
                             tkv[pos] = c[u];
                         }
                     }
                 }
-                __syncthreads();   // bend[b] = end of bucket b
+                __syncthreads();   // bend[b + 1] = end of bucket b
                 // duplicate R keys (a2a_srj_build.c:265-271): each entry against
                 // the later entries of its bucket, in phases (keys, bucket ends,
                 // next entry) so that the LDS reads of all entries overlap;
@@ -1089,7 +1092,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
                     }
 #pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb)] : 0u;
+                    for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = u * NT + threadIdx.x;
@@ -1114,23 +1117,27 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     uint32_t hitm = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
+                        // branch-free: key 0 and padding tuples simply find
+                        // no entry (the table holds no key 0), nor does a key
+                        // of another sub-table of a split partition; table
+                        // keys are unique (duplicates are an error), so at
+                        // most one entry matches
                         const uint32_t k = v[j].x;
+                        const uint32_t bk = bucket_of(k, lb);
+                        const uint32_t st = bend[bk], n = bend[bk + 1] - st;
                         bool hit = false;
                         uint32_t rr = 0;
-                        if (k != kEmptyKey && (Q == 1 || sub_of(sfh(k), I.bits, Q) == q)) {
-                            const uint32_t bk = bucket_of(k, lb);
-                            const uint32_t st = bk ? bend[bk - 1] : 0u, n = bend[bk] - st;
-                            // the first kScan entries without a data-dependent loop
-                            // (the table has kScan padding slots); a longer bucket
-                            // goes on in the tail loop
 #pragma unroll
-                            for (uint32_t u = 0; u < kScan; u++) {
-                                const uint2 e = tkv[st + u];
-                                if (u < n && e.x == k && !hit) { rr = e.y; hit = true; }
-                            }
-                            for (uint32_t i = st + kScan; i < st + n && !hit; i++) {
+                        for (uint32_t u = 0; u < kScan; u++) {
+                            const uint2 e = tkv[st + u];   // the table has kScan padding slots
+                            const bool m = (u < n) && (e.x == k);
+                            rr = m ? e.y : rr;
+                            hit = hit || m;
+                        }
+                        if (n > kScan && !hit) {   // a long bucket (rare)
+                            for (uint32_t i = st + kScan; i < st + n; i++) {
                                 const uint2 e = tkv[i];
-                                if (e.x == k) { rr = e.y; hit = true; }
+                                if (e.x == k) { rr = e.y; hit = true; break; }
                             }
                         }
                         v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
@@ -1223,7 +1230,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 
 size_t lds_join()
 {
-    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + kBuckets * 2;
+    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 2) * 2;
     size_t smj = ((size_t)kSortR + kSortS) * 8 + kJoinNT * 4;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
