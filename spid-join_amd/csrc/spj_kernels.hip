@@ -216,7 +216,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
 // with a heavy table, heavy tuples are not counted (they never go to a
 // partition).  Key 0 is skipped everywhere.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void k_hist(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
+__global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
                                               uint32_t ntiles, uint32_t *__restrict__ counts)
 {
     extern __shared__ uint32_t lds[];
@@ -1255,7 +1255,7 @@ hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t nt
     size_t l = lds_hist(max_bits);
     hipError_t e = set_lds((const void *)k_hist, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(512), l, st, insts, tiles, ntiles, counts);
+    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(1024), l, st, insts, tiles, ntiles, counts);
     return hipGetLastError();
 }
 
