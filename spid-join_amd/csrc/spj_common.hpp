@@ -10,8 +10,9 @@
 //   Rp, Sp               radix-partitioned copies, one region per instance,
 //                        partition p of an instance occupying
 //                        [pstart[p], pstart[p+1]) of its region
-//   counts               per-tile digit counts, scanned in place into the
-//                        tile's global write offsets
+//   counts               per (instance, relation, scatter group) digit
+//                        counts, scanned in place into the group's start
+//                        offsets, then advanced as write cursors
 //   out                  (R rid, S rid) pairs, one region per instance with
 //                        capacity |S_i|; exact count in out_count[i]
 #pragma once
@@ -98,12 +99,16 @@ struct HeavyJob {
 };
 constexpr uint32_t kPreSample = 16;
 
-// One unit of join work: partition p of instance inst, S range [sb, se)
-// (absolute positions in the instance's partitioned S region).
+// One join work item: partition `part` of instance `inst`, S range
+// [sb, se) and the partition's R range [rb, re), absolute positions in the
+// instance's partitioned regions (so the join reads one descriptor, not the
+// partition starts as well).
 struct WorkItem {
     uint32_t inst;
     uint32_t part;
     uint32_t sb, se;
+    uint32_t rb, re;
+    uint32_t pad[2];
 };
 
 }  // namespace spj

@@ -727,6 +727,9 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
             w.part = p;
             w.sb = ps[p] + k * smax;
             w.se = min(ps[p + 1], w.sb + smax);
+            w.rb = pr[p];
+            w.re = pr[p + 1];
+            w.pad[0] = w.pad[1] = 0;
             items[o + k] = w;
         }
         __syncthreads();
@@ -858,13 +861,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         __syncthreads();
         const uint32_t it = sh.item;
         if (it >= total_items) break;
-        // the next item's index is taken now and published at the end of
-        // this item, so the queue's round trip hides behind this item
+        // the next item's index is taken now and published later in this
+        // item (after the table scan on the hash path, else at the end), so
+        // the queue's round trip hides behind this item
         uint32_t next_item = 0;
         if (threadIdx.x == 0) next_item = atomicAdd(&ctrl[1], 1u);
         const WorkItem w = items[it];
         const InstDesc &I = insts[w.inst];
-        const uint32_t rb = pstart[I.pstartR + w.part], re = pstart[I.pstartR + w.part + 1];
+        const uint32_t rb = w.rb, re = w.re;
         const uint32_t nr = re - rb;
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
@@ -1041,7 +1045,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 __syncthreads();
                 const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
-                if (q == 0 && threadIdx.x == 0) sh.base = res_base;   // read after the barriers below
+                if (q == 0 && threadIdx.x == 0) {
+                    // both atomics issued at the item's start have long
+                    // returned; publishing them here rather than at the
+                    // item's end keeps the end from waiting (vector memory
+                    // completes in order) for the probe's stores
+                    sh.base = res_base;   // read after the barriers below
+                    sh.item = next_item;  // read at the next item's start
+                }
                 if (ntab > kTableCap) {
                     if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
                     continue;   // uniform: ntab is the same in every thread
@@ -1159,7 +1170,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 const uint32_t h = atomicAdd(&ctrl[2], 1u);
                 if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
             }
-            sh.item = next_item;
+            if (alg != 0) sh.item = next_item;   // the hash path published it after its scan
         }
     }
 }
