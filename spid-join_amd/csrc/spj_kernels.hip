@@ -418,8 +418,8 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
     uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
     uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
-    uint32_t *bs = lst + FM + 2;                                    // FM: write base - start, this round
-    uint32_t *wtmp = bs + FM;                                       // NT/64 + 1
+    uint32_t *bs = lst;                                             // after staging: write base - start
+    uint32_t *wtmp = lst + FM + 2;                                  // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
 
     for (uint32_t i = threadIdx.x; i < FM + 2; i += NT) lst[i] = 0;
@@ -539,6 +539,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // at the next round's start and does not wait there -- a wait that
         // would also cover this round's stores, issued after them.
         __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();   // staging has read the starts; lst[d] becomes digit d's base - start
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
             const uint32_t d = u * NT + threadIdx.x;
@@ -552,6 +553,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         __syncthreads();
         scatter_write_out(stage, bs, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
                           &err[td.inst]);
+        __syncthreads();   // the write-out has read the bases
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         if (!more) break;
         t = tn;
@@ -1235,7 +1237,7 @@ size_t lds_hist(uint32_t max_bits)
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
-    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + (F + 2) * 4 + F * 4 +
+    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + (F + 2) * 4 +
            (kScatterNT / kWave + 1) * 4 + 16;
 }
 
