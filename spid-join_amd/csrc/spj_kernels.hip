@@ -82,10 +82,9 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_bucket
 }
 
 // Heavy table: kHeavySlots / 4 buckets of 4 slots; a key lives in one of
-// two buckets (2-choice), so a lookup is at most two 16-byte LDS reads and
-// never loops.
+// two buckets (2-choice), so a lookup is two independent 16-byte LDS reads
+// and never loops.
 constexpr uint32_t kHeavyBucketBits = 9;   // 512 buckets x 4 slots = kHeavySlots
-constexpr uint32_t kHeavyFilterLog2 = 16;  // 64K-bit (8 KB) membership bitmap
 __device__ __forceinline__ uint32_t heavy_b1(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - kHeavyBucketBits); }
 __device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
 {
@@ -162,43 +161,34 @@ __device__ __forceinline__ uint32_t heavy_keys_of(const InstDesc &I)
     return *reinterpret_cast<const_u32 *>(reinterpret_cast<uintptr_t>(I.heavy_count));
 }
 
-// Load the instance's heavy table into LDS and build its membership bitmap
-// (kHeavyFilterBits bits; a key whose bit is clear is not heavy).
-__device__ __forceinline__ uint32_t heavy_bit(uint32_t key) { return (key * 0xC2B2AE35u) >> (32 - kHeavyFilterLog2); }
-
-__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr, uint32_t *hf)
+// Load the instance's heavy table (keys, R rids) into LDS; the caller
+// synchronises before using it.
+__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr)
 {
-    for (uint32_t i = threadIdx.x; i < (1u << kHeavyFilterLog2) / 32; i += blockDim.x) hf[i] = 0;
-    __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        const uint32_t k = gld32(I.heavy_keys, i);
-        hk[i] = k;
+        hk[i] = gld32(I.heavy_keys, i);
         hr[i] = gld32(I.heavy_rids, i);
-        if (k != kEmptyKey) {
-            const uint32_t b = heavy_bit(k);
-            atomicOr(&hf[b >> 5], 1u << (b & 31u));
-        }
     }
 }
 
-// Slot of key (never 0) in the LDS heavy table, or -1.  hk is 16-byte aligned.
-__device__ __forceinline__ int heavy_find(const uint32_t *hk, const uint32_t *hf, uint32_t key)
+// Slot of key (never 0: empty slots hold key 0) in the LDS heavy table, or
+// -1.  Both candidate buckets are read at once and the slot is selected
+// without branches.  hk is 16-byte aligned.
+__device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key)
 {
-    const uint32_t f = heavy_bit(key);
-    if (!((hf[f >> 5] >> (f & 31u)) & 1u)) return -1;
     const uint4 *hb = reinterpret_cast<const uint4 *>(hk);
     const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
-    const uint4 a = hb[b1];
-    if (a.x == key) return (int)(4 * b1);
-    if (a.y == key) return (int)(4 * b1 + 1);
-    if (a.z == key) return (int)(4 * b1 + 2);
-    if (a.w == key) return (int)(4 * b1 + 3);
-    const uint4 c = hb[b2];
-    if (c.x == key) return (int)(4 * b2);
-    if (c.y == key) return (int)(4 * b2 + 1);
-    if (c.z == key) return (int)(4 * b2 + 2);
-    if (c.w == key) return (int)(4 * b2 + 3);
-    return -1;
+    const uint4 a = hb[b1], c = hb[b2];
+    int s = -1;
+    s = (c.w == key) ? (int)(4 * b2 + 3) : s;
+    s = (c.z == key) ? (int)(4 * b2 + 2) : s;
+    s = (c.y == key) ? (int)(4 * b2 + 1) : s;
+    s = (c.x == key) ? (int)(4 * b2) : s;
+    s = (a.w == key) ? (int)(4 * b1 + 3) : s;
+    s = (a.z == key) ? (int)(4 * b1 + 2) : s;
+    s = (a.y == key) ? (int)(4 * b1 + 1) : s;
+    s = (a.x == key) ? (int)(4 * b1) : s;
+    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -226,11 +216,10 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
     const uint32_t F = 1u << I.bits, mask = F - 1u;
     uint32_t *hk = lds;                   // kHeavySlots (16-byte aligned)
     uint32_t *hr = hk + kHeavySlots;      // kHeavySlots
-    uint32_t *hf = hr + kHeavySlots;      // heavy bitmap
-    uint32_t *hist = hf + (1u << kHeavyFilterLog2) / 32;   // F
+    uint32_t *hist = hr + kHeavySlots;    // F
     const bool heavy = I.heavy != 0 && heavy_keys_of(I) != 0;
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
-    if (heavy) load_heavy(I, hk, hr, hf);
+    if (heavy) load_heavy(I, hk, hr);
     __syncthreads();
     const uint2 *in = td.rel ? I.S : I.R;
     const uint32_t mode = !heavy ? 0u : (td.rel == 0 ? 1u : 2u);
@@ -245,7 +234,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
         // heavy lookups of all U tuples first, so their LDS reads overlap
         int hs[U];
 #pragma unroll
-        for (uint32_t j = 0; j < U; j++) hs[j] = (mode != 0 && v[j].x != kEmptyKey) ? heavy_find(hk, hf, v[j].x) : -1;
+        for (uint32_t j = 0; j < U; j++) hs[j] = (mode != 0 && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
 #pragma unroll
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
@@ -415,8 +404,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     // LDS layout
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
     uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
-    uint32_t *hf = hr + kHeavySlots;                                // heavy bitmap
-    uint2 *stage = reinterpret_cast<uint2 *>(hf + (1u << kHeavyFilterLog2) / 32);   // ROUND tuples
+    uint2 *stage = reinterpret_cast<uint2 *>(hr + kHeavySlots);     // ROUND tuples
     uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
     uint32_t *bs = lst;                                             // after staging: write base - start
     uint32_t *wtmp = lst + FM + 2;                                  // NT/64 + 1
@@ -453,7 +441,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         const uint32_t rn = min(ROUND, td.end - o);
         __syncthreads();   // the previous round's write-out and clearing are done
         if (heavy && heavy_inst != td.inst) {
-            load_heavy(I, hk, hr, hf);
+            load_heavy(I, hk, hr);
             heavy_inst = td.inst;
             __syncthreads();
         }
@@ -466,7 +454,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
-                hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, hf, v[j].x) : -1;
+                hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
             }
             uint32_t hrid[IPT];
 #pragma unroll
@@ -1231,13 +1219,13 @@ static hipError_t set_lds(const void *fn, size_t bytes)
 
 size_t lds_hist(uint32_t max_bits)
 {
-    return ((size_t(1) << max_bits) + 2 * kHeavySlots + (1u << kHeavyFilterLog2) / 32) * 4;
+    return ((size_t(1) << max_bits) + 2 * kHeavySlots) * 4;
 }
 
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
-    return 2 * kHeavySlots * 4 + (1u << kHeavyFilterLog2) / 8 + R * 8 + (F + 2) * 4 +
+    return 2 * kHeavySlots * 4 + R * 8 + (F + 2) * 4 +
            (kScatterNT / kWave + 1) * 4 + 16;
 }
 
