@@ -567,7 +567,7 @@ constexpr uint32_t kSampleSlots = 16384;
 constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
 
-__global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
+__global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                       uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase)
 {
     extern __shared__ uint32_t lds[];
@@ -593,13 +593,13 @@ __global__ __launch_bounds__(512) void k_heavy_sample(const InstDesc *__restrict
     __syncthreads();
     const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
     const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
-    // all of the block's keys are loaded first (kSampleBlock / 512 per
+    // all of the block's keys are loaded first (kSampleBlock / 1024 per
     // thread), so the loads overlap instead of waiting one by one
-    constexpr uint32_t KPT = kSampleBlock / 512;
+    constexpr uint32_t KPT = kSampleBlock / 1024;
     uint32_t keys[KPT];
 #pragma unroll
     for (uint32_t u = 0; u < KPT; u++) {
-        const uint64_t i = b + u * 512 + threadIdx.x;
+        const uint64_t i = b + u * 1024 + threadIdx.x;
         keys[u] = (i < e) ? gld(I.S, i).x : kEmptyKey;
     }
 #pragma unroll
@@ -1324,8 +1324,8 @@ hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint
     size_t l = 2 * kSampleSlots * 4;
     hipError_t e = set_lds((const void *)k_heavy_sample, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[0]), dim3(512), l, st, insts, jobs, njobs, cand_base, 0u);
-    if (blocks[1]) hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[1]), dim3(512), l, st, insts, jobs, njobs, cand_base, 1u);
+    hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[0]), dim3(1024), l, st, insts, jobs, njobs, cand_base, 0u);
+    if (blocks[1]) hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[1]), dim3(1024), l, st, insts, jobs, njobs, cand_base, 1u);
     hipLaunchKernelGGL(k_heavy_select, dim3(njobs), dim3(1024), 0, st, insts, jobs, cand_base, nheavy);
     return hipGetLastError();
 }
