@@ -284,9 +284,11 @@ def serving(args):
     tids = [sv.add_table(R) for R in tables]
     log(f"[bench] serving: {ntab} tables (sum |R| {int(sizes.sum())}), {len(pool)} requests "
         f"(sum |S| {sum(len(s) for _, s in pool)}); ready in {time.time() - t0:.1f}s")
-    # warm-up
-    for t, S in pool[:8]:
-        sv.wait(sv.submit(tids[t], S))
+    # warm-up: the largest requests, submitted together so that every slot
+    # runs some and sizes its buffers before the timed sweep
+    big = sorted(range(len(pool)), key=lambda k: -len(pool[k][1]))[:2 * args.slots]
+    for rid in [sv.submit(tids[pool[k][0]], pool[k][1]) for k in big]:
+        sv.wait(rid)
     results = []
     for rate in args.rates:
         gaps = rng.exponential(1.0 / rate, len(pool))

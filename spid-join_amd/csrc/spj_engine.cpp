@@ -131,6 +131,10 @@ Context::Context()
     (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     for (auto &e : ev_) (void)hipEventCreate(&e);
     (void)hipEventCreateWithFlags(&desc_ev_, hipEventDisableTiming);
+    // page-locked staging is slow to allocate: take a size that covers
+    // most batches now, not in the first join (a latency outlier)
+    (void)pin_desc_.ensure(size_t(1) << 20);
+    (void)pin_ctrl_.ensure(size_t(64) << 10);
 }
 
 Context::~Context()
