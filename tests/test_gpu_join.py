@@ -109,6 +109,28 @@ def test_batch_matches_single(gpu, orc):
         check_equal(sj, orc, R, S, p)
 
 
+def test_batch_many_tiny_and_empty(gpu, orc):
+    """Hundreds of instances, many of them tiny or empty on one side, next to
+    a skewed one with heavy keys: tiles of many instances share the scatter
+    groups and the join's work queue."""
+    sj = gpu
+    rng = np.random.default_rng(77)
+    Rs, Ss = [], []
+    for i in range(300):
+        nR = int(rng.choice([0, 1, 2, 7, 33, 100, 1000, 5000]))
+        nS = int(rng.choice([0, 1, 3, 50, 999, 4096, 20000]))
+        R = sj.gen_r(nR, 3000 + i)
+        S = sj.gen_s(nS, R, float(rng.uniform(0, 1.5)), 4000 + i) if nR else np.zeros(0, dtype=sj.TUPLE)
+        Rs.append(R)
+        Ss.append(S)
+    Rs.append(sj.gen_r(1 << 18, 5000))
+    Ss.append(sj.gen_s(1 << 21, Rs[-1], 1.5, 5001))
+    print("case batch many tiny", flush=True)
+    outs, _ = sj.join_batch(Rs, Ss)
+    for R, S, p in zip(Rs, Ss, outs):
+        check_equal(sj, orc, R, S, p)
+
+
 def test_join_instance_lifecycle(gpu, orc):
     """LoadColumn / PrepareJoin / StartJoin (async) / FinishJoin, query.cc."""
     sj = gpu
