@@ -55,6 +55,11 @@ typedef struct {
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
     int64_t tile_tuples;      /* tuples per partition-pass tile */
     int64_t items_s_max;      /* S tuples per join work item */
+    int32_t part_layout;      /* partitions of R and S: 0 counted (a histogram pass sizes them
+                                 exactly), 1 cells (no histogram pass: fixed-capacity cells per
+                                 scatter group and digit; the join is redone counted when a cell
+                                 overflows); AUTO = cells for a hash-join relation of >= 2^20 tuples */
+    int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
 } spj_config_t;
 
 /* Per-phase device time of the last call (HIP events on its stream) and, for
@@ -64,6 +69,8 @@ typedef struct {
     float heavy_ms, hist_ms, scan_ms, scatter_ms, items_ms, join_ms, total_ms;
     uint32_t heavy_keys;
     uint64_t r_partitioned, s_partitioned;
+    uint32_t s_cells;   /* 1: S went to fixed-capacity cells (no histogram pass), 0: counted */
+    uint32_t r_cells;   /* the same for R */
 } spj_phase_times_t;
 
 void spj_config_default(spj_config_t *cfg);

@@ -57,6 +57,8 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
     if (cfg->scatter_ipt >= 0) r.ipt = (uint32_t)cfg->scatter_ipt;
     if (cfg->tile_tuples > 0) r.tile = (uint32_t)cfg->tile_tuples;
     if (cfg->items_s_max > 0) r.smax = (uint32_t)cfg->items_s_max;
+    if (cfg->part_layout >= 0) r.cells = (uint32_t)cfg->part_layout;
+    if (cfg->cell_capacity_pct >= 0) r.cell_pct = (uint32_t)cfg->cell_capacity_pct;
     return r;
 }
 
@@ -69,6 +71,8 @@ int check_resolved(const Resolved &r)
         return fail(SPJ_EINVAL, "scatter_ipt %u (4, 8, 10 or 12)", r.ipt);
     if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);
     if (r.smax < 512) return fail(SPJ_EINVAL, "items_s_max %u < 512", r.smax);
+    if (r.cells > 2) return fail(SPJ_EINVAL, "part_layout %u (0 counted, 1 cells)", r.cells);
+    if (r.cell_pct > 200) return fail(SPJ_EINVAL, "cell_capacity_pct %u > 200", r.cell_pct);
     return SPJ_OK;
 }
 
@@ -96,6 +100,8 @@ void fill_times(const PhaseTimes &t, spj_phase_times_t *o)
     o->heavy_keys = t.heavy_keys;
     o->r_partitioned = t.r_parted;
     o->s_partitioned = t.s_parted;
+    o->s_cells = t.s_cells;
+    o->r_cells = t.r_cells;
 }
 
 int validate_sizes(uint64_t nR, uint64_t nS)
@@ -123,6 +129,8 @@ void spj_config_default(spj_config_t *cfg)
     cfg->scatter_ipt = -1;
     cfg->tile_tuples = -1;
     cfg->items_s_max = -1;
+    cfg->part_layout = -1;
+    cfg->cell_capacity_pct = -1;
 }
 
 const char *spj_last_error(void) { return g_err.c_str(); }

@@ -13,6 +13,11 @@
 //   counts               per (instance, relation, scatter group) digit
 //                        counts, scanned in place into the group's start
 //                        offsets, then advanced as write cursors
+//   cells                (large relations, no histogram pass) group g's
+//                        tuples of digit d go to a cell of fixed capacity
+//                        cap_g at cell0_g + d * cap_g of the region; the
+//                        cursor starts at the cell's base and ends at its
+//                        fill
 //   out                  (R rid, S rid) pairs, one region per instance with
 //                        capacity |S_i|; exact count in out_count[i]
 #pragma once
@@ -28,11 +33,13 @@ constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
 constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
 constexpr uint32_t kScatterGroups = 8;      // scatter groups (one per XCD) per relation
+constexpr uint32_t kSinkTuples = kScatterNT * 12;   // largest scatter round
 
 // error bits in err[inst]
 constexpr uint32_t kErrDuplicateKey = 1u;   // a2a_srj_build.c:265-271 (VALIDATION)
 constexpr uint32_t kErrTableOverflow = 2u;  // a sub-table received more tuples than slots
 constexpr uint32_t kErrOutOverflow = 4u;    // more pairs than the output capacity
+constexpr uint32_t kErrCellOverflow = 8u;   // an S cell received more tuples than its capacity
 
 struct InstDesc {
     const uint2 *R;          // input R (device)
@@ -50,16 +57,43 @@ struct InstDesc {
     const uint32_t *heavy_count;   // number of heavy keys found (device)
     uint32_t table_log2;     // LDS hash table slots (log2) for this instance
     uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop
+    // Cell layout of S (s_) and R (r_): 1 + index of the relation's first
+    // of kScatterGroups CellGroups (0 = counted layout), the region offset
+    // of a kSinkTuples sink for runs that overflow a cell, and the index of
+    // partition 0's SMap entry.
+    uint32_t s_cells, s_sink, s_map;
+    uint32_t r_cells, r_sink, r_map;
+};
+
+// Cell layout of one scatter group of one relation: the group's cursors
+// at cnt_off (F of them), digit d's cell at cell0 + d * cap of the Sp
+// region (cap = 0: the group holds no S tile of the instance).
+struct CellGroup {
+    uint64_t cnt_off;
+    uint32_t cell0;
+    uint32_t cap;
+    uint32_t bits;
+    uint32_t pad;
+};
+
+// Where a partition's tuples lie in the cell layout: partition-relative
+// position v is at region[v + D[g]] for the largest g with V[g] <= v
+// (V[0] = 0; V[g] = total of cells 0..g-1).  The contiguous layout is
+// V = {0, ~0, ...}, D = {0}.
+struct SMap {
+    uint32_t V[kScatterGroups];
+    uint32_t D[kScatterGroups];
 };
 
 // One tile of one relation of one instance.  cnt_off is the offset of the
 // F digit counters of the tile's scatter group: histogram counts before the
-// scan, write cursors after it.
+// scan, write cursors after it (cell layout: cursors from the start).
 struct TileDesc {
     uint32_t inst;
     uint32_t rel;            // 0 = R, 1 = S
     uint32_t begin, end;     // tuple range inside the relation
     uint64_t cnt_off;
+    uint32_t cell0, cap;     // cell layout: the group's cells (cap = 0: counted layout)
 };
 
 // One (instance, relation) whose group counts are scanned together: t_count
@@ -102,13 +136,16 @@ constexpr uint32_t kPreSample = 16;
 // One join work item: partition `part` of instance `inst`, S range
 // [sb, se) and the partition's R range [rb, re), absolute positions in the
 // instance's partitioned regions (so the join reads one descriptor, not the
-// partition starts as well).
+// partition starts as well).  With a cell layout, the relation's range
+// holds partition-relative positions mapped through SMap entry smap - 1
+// (S) or rmap - 1 (R).
 struct WorkItem {
     uint32_t inst;
     uint32_t part;
     uint32_t sb, se;
     uint32_t rb, re;
-    uint32_t pad[2];
+    uint32_t smap;           // 0: contiguous S
+    uint32_t rmap;           // 0: contiguous R
 };
 
 }  // namespace spj

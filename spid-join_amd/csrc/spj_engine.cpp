@@ -9,6 +9,7 @@
 #include "spj_engine.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "spj_kernels.hpp"
@@ -144,7 +145,7 @@ Context::~Context()
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
     DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
-                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_};
+                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &smap_};
     for (auto *b : bufs) b->release();
     pin_desc_.release();
     pin_ctrl_.release();
@@ -154,16 +155,34 @@ Context::~Context()
 // ctrl layout (uint32): [0] nitems, [1] qhead, [2] holes recorded, [3] pad,
 // then n words each of: out_lo (pairs reserved by join items, from the front),
 // out_hi (heavy pairs, from the back), out_fill (pairs written by items),
-// err, nheavy.
-enum { kCtrlLo = 0, kCtrlHi = 1, kCtrlFill = 2, kCtrlErr = 3, kCtrlHeavy = 4, kCtrlArrays = 5 };
+// err, nheavy, spart and rpart (S and R tuples in partitions).
+enum {
+    kCtrlLo = 0, kCtrlHi = 1, kCtrlFill = 2, kCtrlErr = 3, kCtrlHeavy = 4, kCtrlSPart = 5, kCtrlRPart = 6,
+    kCtrlArrays = 7
+};
 static inline size_t ctrl_words(size_t n) { return 4 + kCtrlArrays * n; }
 static inline size_t ctrl_at(size_t n, int which, size_t i) { return 4 + (size_t)which * n + i; }
 
-hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed)
+// Capacity of one S cell that takes a group's tuples of one digit, from the
+// group's tuple count n and the fan-out F: by default 1.5 times the mean
+// fill plus 8 standard deviations of a uniform spread plus 64, so that only
+// keys too frequent for the heavy detection to have missed overflow a cell.
+static uint32_t cell_capacity(uint64_t n, uint32_t F, uint32_t pct)
 {
+    if (!n) return 0;
+    const double m = (double)n / F;
+    if (pct) return (uint32_t)std::ceil(m * pct / 100.0) + 1u;
+    return (uint32_t)std::ceil(1.5 * m + 8.0 * std::sqrt(m) + 64.0);
+}
+
+hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *timed)
+{
+    if (&specs_in != &h_specs_) h_specs_ = specs_in;
+    const std::vector<InstSpec> &specs = h_specs_;
     const size_t n = specs.size();
     h_insts_.assign(n, InstDesc{});
     h_tiles_.clear();
+    h_cells_.clear();
     h_segs_.clear();
     h_ad_.clear();
     h_b_.clear();
@@ -173,12 +192,21 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     any_heavy_ = false;
 
     // ---- plan: regions, tiles, scan segments
+    // Tiles are listed as [R cells | R counted | S counted | S cells]: the
+    // histogram pass reads the counted ones (a contiguous range); when R
+    // has cells and heavy keys, R is scattered by a launch of its own (its
+    // tiles record the heavy keys' R rids that the S scatter emits).
     uint64_t rp_total = 0, sp_total = 0, cnt_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
+    uint64_t smap_total = 0;
     // one S-range size per batch (the item kernel takes a single value)
     const uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
     uint32_t nheavy_inst = 0;
     std::vector<uint64_t> rp_off(n), sp_off(n);
     std::vector<int> heavy_idx(n, -1);
+    std::vector<char> cells[2] = {std::vector<char>(n, 0), std::vector<char>(n, 0)};
+    std::vector<uint64_t> cnt0[2] = {std::vector<uint64_t>(n, 0), std::vector<uint64_t>(n, 0)};
+    std::vector<TileDesc> lists[4];   // R cells, R counted, S counted, S cells
+    bool split = false;
     for (size_t i = 0; i < n; i++) {
         const InstSpec &s = specs[i];
         const Resolved &c = s.cfg;
@@ -187,10 +215,14 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         max_table_ = std::max(max_table_, c.table_log2);
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
         const bool pre = s.Rp_pre != nullptr;
-        rp_off[i] = rp_total;
-        sp_off[i] = sp_total;
-        if (!pre) rp_total += s.nR;
-        sp_total += s.nS;
+        // cells: no histogram pass.  Cursors are 32-bit, so a relation's
+        // cell region plus all its tuples stays below 2^32.
+        auto use_cells = [&](uint32_t len) {
+            return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
+                   (c.cells == 1 || (c.cells == 2 && c.alg == 0 && len >= (1u << 20)));
+        };
+        cells[0][i] = !pre && !s.r_contiguous && use_cells(s.nR);
+        cells[1][i] = use_cells(s.nS);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
@@ -207,21 +239,28 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         if (d.heavy) {
             heavy_idx[i] = (int)nheavy_inst++;
             any_heavy_ = true;
+            if (cells[0][i]) split = true;
         }
         items_bound += F + (uint64_t)s.nS / smax + 1;
         for (uint32_t rel = pre ? 1u : 0u; rel < 2; rel++) {
             const uint32_t len = rel ? s.nS : s.nR;
             const uint32_t T = (len + c.tile - 1) / c.tile;
-            ScanSeg sg{};
-            sg.inst = (uint32_t)i;
-            sg.rel = rel;
-            sg.bits = c.bits;
-            sg.t_count = kScatterGroups;
-            sg.cnt_off0 = cnt_total;
-            sg.part_off = part_total;
-            sg.pstart_off = rel ? d.pstartS : d.pstartR;
-            const uint32_t seg_id = (uint32_t)h_segs_.size();
-            h_segs_.push_back(sg);
+            const bool cl = cells[rel][i];
+            cnt0[rel][i] = cnt_total;
+            uint32_t seg_id = 0;
+            if (!cl) {
+                ScanSeg sg{};
+                sg.inst = (uint32_t)i;
+                sg.rel = rel;
+                sg.bits = c.bits;
+                sg.t_count = kScatterGroups;
+                sg.cnt_off0 = cnt_total;
+                sg.part_off = part_total;
+                sg.pstart_off = rel ? d.pstartS : d.pstartR;
+                seg_id = (uint32_t)h_segs_.size();
+                h_segs_.push_back(sg);
+            }
+            std::vector<TileDesc> &dst = lists[rel == 0 ? (cl ? 0 : 1) : (cl ? 3 : 2)];
             for (uint32_t t = 0; t < T; t++) {
                 TileDesc td{};
                 td.inst = (uint32_t)i;
@@ -229,27 +268,84 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
                 td.begin = t * c.tile;
                 td.end = std::min<uint64_t>(len, (uint64_t)(t + 1) * c.tile);
                 td.cnt_off = cnt_total;   // + group * F below
-                h_tiles_.push_back(td);
+                dst.push_back(td);
             }
-            const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
-            for (uint32_t dc = 0; dc < dch; dc++) {
-                h_b_.push_back(ScanBlock{seg_id, dc, 0, 0});
-                for (uint32_t tc = 0; tc < tch; tc++) h_ad_.push_back(ScanBlock{seg_id, dc, tc, 0});
+            if (!cl) {
+                const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
+                for (uint32_t dc = 0; dc < dch; dc++) {
+                    h_b_.push_back(ScanBlock{seg_id, dc, 0, 0});
+                    for (uint32_t tc = 0; tc < tch; tc++) h_ad_.push_back(ScanBlock{seg_id, dc, tc, 0});
+                }
+                part_total += (uint64_t)tch * F;
             }
             cnt_total += (uint64_t)kScatterGroups * F;
-            part_total += (uint64_t)tch * F;
         }
     }
+    const uint32_t hist_first = (uint32_t)lists[0].size();
+    const uint32_t n_counted = (uint32_t)(lists[1].size() + lists[2].size());
+    for (auto &l : lists) h_tiles_.insert(h_tiles_.end(), l.begin(), l.end());
     if (h_tiles_.size() > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    // scatter launches: [0, scat_split) and [scat_split, T)
+    const uint32_t scat_split = split ? (uint32_t)(lists[0].size() + lists[1].size()) : 0u;
     // scatter groups: the tiles of XCD x's range (as the scatter kernel walks
-    // them) share group x's counters
-    {
-        const uint64_t T = h_tiles_.size(), q = T / kScatterGroups, r = T % kScatterGroups;
-        for (uint64_t t = 0; t < T; t++) {
-            const uint64_t g = (t < r * (q + 1)) ? t / (q + 1) : r + (t - r * (q + 1)) / q;
-            TileDesc &td = h_tiles_[t];
+    // a launch's tiles) share group x's counters
+    std::vector<uint8_t> tgroup(h_tiles_.size());
+    auto assign_groups = [&](uint64_t t0, uint64_t t1) {
+        const uint64_t T = t1 - t0, q = T / kScatterGroups, r = T % kScatterGroups;
+        for (uint64_t k = 0; k < T; k++) {
+            const uint64_t g = (k < r * (q + 1)) ? k / (q + 1) : r + (k - r * (q + 1)) / q;
+            TileDesc &td = h_tiles_[t0 + k];
             td.cnt_off += g * (uint64_t(1) << h_insts_[td.inst].bits);
+            tgroup[t0 + k] = (uint8_t)g;
         }
+    };
+    assign_groups(0, scat_split);
+    assign_groups(scat_split, h_tiles_.size());
+    // regions: contiguous, or cells sized by each group's share
+    std::vector<uint64_t> gsize((size_t)n * 2 * kScatterGroups, 0);
+    auto gk = [](size_t i, uint32_t rel, uint32_t g) { return (i * 2 + rel) * kScatterGroups + g; };
+    for (uint64_t t = 0; t < h_tiles_.size(); t++) {
+        const TileDesc &td = h_tiles_[t];
+        if (cells[td.rel][td.inst]) gsize[gk(td.inst, td.rel, tgroup[t])] += td.end - td.begin;
+    }
+    std::vector<uint32_t> gcell0(gsize.size(), 0), gcap(gsize.size(), 0);
+    for (size_t i = 0; i < n; i++) {
+        InstDesc &d = h_insts_[i];
+        const uint32_t F = 1u << d.bits;
+        for (uint32_t rel = 0; rel < 2; rel++) {
+            uint64_t &total = rel ? sp_total : rp_total;
+            (rel ? sp_off : rp_off)[i] = total;
+            if (!cells[rel][i]) {
+                if (rel == 1 || !specs[i].Rp_pre) total += rel ? d.nS : d.nR;
+                continue;
+            }
+            uint64_t region = 0;
+            const uint32_t first = (uint32_t)h_cells_.size() + 1;
+            for (uint32_t g = 0; g < kScatterGroups; g++) {
+                const size_t k = gk(i, rel, g);
+                gcap[k] = cell_capacity(gsize[k], F, specs[i].cfg.cell_pct);
+                gcell0[k] = (uint32_t)region;
+                region += (uint64_t)F * gcap[k];
+                CellGroup cg{};
+                cg.cnt_off = cnt0[rel][i] + (uint64_t)g * F;
+                cg.cell0 = gcell0[k];
+                cg.cap = gcap[k];
+                cg.bits = d.bits;
+                h_cells_.push_back(cg);
+            }
+            (rel ? d.s_cells : d.r_cells) = first;
+            (rel ? d.s_sink : d.r_sink) = (uint32_t)region;
+            (rel ? d.s_map : d.r_map) = (uint32_t)smap_total;
+            smap_total += F;
+            total += region + kSinkTuples;
+        }
+    }
+    for (uint64_t t = 0; t < h_tiles_.size(); t++) {
+        TileDesc &td = h_tiles_[t];
+        if (!cells[td.rel][td.inst]) continue;
+        const size_t k = gk(td.inst, td.rel, tgroup[t]);
+        td.cell0 = gcell0[k];
+        td.cap = gcap[k];
     }
 
     // ---- heavy-key jobs (sampling blocks of both phases)
@@ -279,7 +375,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     const size_t o_ad = o_segs + al(h_segs_.size() * sizeof(ScanSeg));
     const size_t o_b = o_ad + al(h_ad_.size() * sizeof(ScanBlock));
     const size_t o_jobs = o_b + al(h_b_.size() * sizeof(ScanBlock));
-    const size_t desc_bytes = o_jobs + al(h_jobs_.size() * sizeof(HeavyJob));
+    const size_t o_cells = o_jobs + al(h_jobs_.size() * sizeof(HeavyJob));
+    const size_t desc_bytes = o_cells + al(h_cells_.size() * sizeof(CellGroup));
     SPJ_TRY(desc_.ensure(desc_bytes));
     SPJ_TRY(counts_.ensure(cnt_total * 4));
     SPJ_TRY(part_.ensure(part_total * 4));
@@ -287,6 +384,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     SPJ_TRY(rp_.ensure(rp_total * 8));
     SPJ_TRY(sp_.ensure(sp_total * 8));
     SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
+    SPJ_TRY(smap_.ensure(smap_total * sizeof(SMap)));
     SPJ_TRY(holes_.ensure(items_bound * sizeof(uint4)));
     max_holes_ = (uint32_t)std::min<uint64_t>(items_bound, 0xFFFFFFFFull);
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
@@ -318,6 +416,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
         if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
         if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
         if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
+        if (!h_cells_.empty()) memcpy(h + o_cells, h_cells_.data(), h_cells_.size() * sizeof(CellGroup));
     }
     SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
     SPJ_TRY(hipEventRecord(desc_ev_, st));
@@ -328,6 +427,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     const ScanBlock *d_ad = reinterpret_cast<const ScanBlock *>(dd + o_ad);
     const ScanBlock *d_b = reinterpret_cast<const ScanBlock *>(dd + o_b);
     const HeavyJob *d_jobs = reinterpret_cast<const HeavyJob *>(dd + o_jobs);
+    const CellGroup *d_cells = reinterpret_cast<const CellGroup *>(dd + o_cells);
     for (size_t i = 0; i < n; i++)   // resident tables: their partition starts
         if (specs[i].Rp_pre)
             SPJ_TRY(hipMemcpyAsync(pstart_.as<uint32_t>() + h_insts_[i].pstartR, specs[i].pst_pre,
@@ -337,7 +437,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     uint32_t *ctrl = ctrl_.as<uint32_t>();
     uint32_t *out_lo = ctrl + ctrl_at(n, kCtrlLo, 0), *out_hi = ctrl + ctrl_at(n, kCtrlHi, 0);
     uint32_t *out_fill = ctrl + ctrl_at(n, kCtrlFill, 0), *err = ctrl + ctrl_at(n, kCtrlErr, 0);
-    uint32_t *nheavy = ctrl + ctrl_at(n, kCtrlHeavy, 0);
+    uint32_t *nheavy = ctrl + ctrl_at(n, kCtrlHeavy, 0), *spart = ctrl + ctrl_at(n, kCtrlSPart, 0);
+    uint32_t *rpart = ctrl + ctrl_at(n, kCtrlRPart, 0);
     if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
     SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
 
@@ -351,26 +452,33 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs, PhaseTimes *time
     }
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 
-    // ---- partition: histogram, scan, scatter
+    // ---- partition: histogram (counted tiles), scan, scatter
     const uint32_t ntiles = (uint32_t)h_tiles_.size();
     SPJ_TRY(launch_fill(counts_.as<uint32_t>(), cnt_total, 0u, st));
-    SPJ_TRY(launch_hist(d_insts, d_tiles, ntiles, counts_.as<uint32_t>(), max_bits_, st));
+    SPJ_TRY(launch_cell_init(d_cells, (uint32_t)h_cells_.size(), counts_.as<uint32_t>(), st));
+    SPJ_TRY(launch_hist(d_insts, d_tiles + hist_first, n_counted, counts_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
     SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), d_ad, (uint32_t)h_ad_.size(), d_b, (uint32_t)h_b_.size(),
                         counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
-    // one persistent 1024-thread workgroup per CU (LDS-bound)
-    SPJ_TRY(launch_scatter(d_insts, d_tiles, ntiles, counts_.as<uint32_t>(), out_hi, err, max_bits_,
-                           max_ipt_, (uint32_t)std::max(1, device_info().cus), st));
+    // one persistent 1024-thread workgroup per CU (LDS-bound); R first
+    // when its tiles record heavy rids for the S tiles
+    const uint32_t sgrid = (uint32_t)std::max(1, device_info().cus);
+    if (scat_split)
+        SPJ_TRY(launch_scatter(d_insts, d_tiles, scat_split, counts_.as<uint32_t>(), out_hi, err, max_bits_, max_ipt_,
+                               sgrid, st));
+    SPJ_TRY(launch_scatter(d_insts, d_tiles + scat_split, ntiles - scat_split, counts_.as<uint32_t>(), out_hi, err,
+                           max_bits_, max_ipt_, sgrid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
     // ---- join
-    SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), smax, items_.as<WorkItem>(), ctrl, st));
+    SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), d_cells, counts_.as<uint32_t>(),
+                              smap_.as<SMap>(), smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
     const DeviceInfo &di = device_info();
     const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
-    SPJ_TRY(launch_join(d_insts, pstart_.as<uint32_t>(), items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
+    SPJ_TRY(launch_join(d_insts, smap_.as<SMap>(), items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
                         holes_.as<uint4>(), max_holes_, grid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
     specs_n_ = n;
@@ -422,6 +530,16 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
     memcpy(h_ctrl_.data(), pin_ctrl_.p, ctrl_words(n) * 4);
     counts.resize(n);
     errs.assign(h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0), h_ctrl_.begin() + ctrl_at(n, kCtrlErr, 0) + n);
+    // an S cell overflowed (keys too frequent for their cells that the heavy
+    // detection did not take): redo the batch with counted partitions
+    if (!force_counted_ && n == h_specs_.size() &&
+        std::any_of(errs.begin(), errs.end(), [](uint32_t e) { return (e & kErrCellOverflow) != 0; })) {
+        force_counted_ = true;
+        hipError_t e = enqueue(h_specs_, timed);
+        if (e == hipSuccess) e = collect(n, counts, errs, timed);
+        force_counted_ = false;
+        return e;
+    }
     std::vector<uint4> holes;
     bool holes_read = false;
     for (size_t i = 0; i < n; i++) {
@@ -455,12 +573,10 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         if (n) {
             timed->heavy_keys = h_ctrl_[ctrl_at(n, kCtrlHeavy, 0)];
             const InstDesc &d0 = h_insts_[0];
-            const uint32_t F = 1u << d0.bits;
-            uint32_t tr = 0, ts = 0;
-            SPJ_TRY(hipMemcpy(&tr, pstart_.as<uint32_t>() + d0.pstartR + F, 4, hipMemcpyDeviceToHost));
-            SPJ_TRY(hipMemcpy(&ts, pstart_.as<uint32_t>() + d0.pstartS + F, 4, hipMemcpyDeviceToHost));
-            timed->r_parted = tr;
-            timed->s_parted = ts;
+            timed->r_parted = h_ctrl_[ctrl_at(n, kCtrlRPart, 0)];
+            timed->s_parted = h_ctrl_[ctrl_at(n, kCtrlSPart, 0)];
+            timed->s_cells = d0.s_cells != 0 ? 1u : 0u;
+            timed->r_cells = d0.r_cells != 0 ? 1u : 0u;
         }
     }
     return hipSuccess;
@@ -468,7 +584,7 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
 
 hipError_t Context::export_partitioned_r(size_t i, uint2 *rp_dst, uint32_t *pstart_dst)
 {
-    if (i >= h_insts_.size()) return hipErrorInvalidValue;
+    if (i >= h_insts_.size() || h_insts_[i].r_cells) return hipErrorInvalidValue;
     const InstDesc &d = h_insts_[i];
     const size_t F = size_t(1) << d.bits;
     if (d.nR) SPJ_TRY(hipMemcpyAsync(rp_dst, d.Rp, (size_t)d.nR * 8, hipMemcpyDeviceToDevice, stream_));

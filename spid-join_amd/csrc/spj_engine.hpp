@@ -22,6 +22,8 @@ struct Resolved {
     uint32_t ipt = 12;        // scatter tuples per thread (round = 1024 * ipt)
     uint32_t tile = 65536;    // tuples per tile of the partition pass
     uint32_t smax = 131072;   // S tuples per work item
+    uint32_t cells = 2;       // S layout: 0 counted (histogram pass), 1 cells (none), 2 auto
+    uint32_t cell_pct = 0;    // cell capacity in % of the mean cell fill (0 = automatic)
 };
 
 struct InstSpec {
@@ -36,6 +38,8 @@ struct InstSpec {
     // partition starts.  Only S is partitioned then; heavy handling is off.
     const uint2 *Rp_pre = nullptr;
     const uint32_t *pst_pre = nullptr;
+    // R must end up in contiguous partitions (export_partitioned_r)
+    bool r_contiguous = false;
 };
 
 // Milliseconds per phase (HIP events on the context's stream).
@@ -43,6 +47,7 @@ struct PhaseTimes {
     float heavy = 0, hist = 0, scan = 0, scatter = 0, items = 0, join = 0, total = 0;
     uint32_t heavy_keys = 0;            // instance 0
     uint64_t r_parted = 0, s_parted = 0;  // instance 0: tuples written to partitions
+    uint32_t s_cells = 0, r_cells = 0;    // instance 0: S / R layout was cells
 };
 
 struct DevBuf {
@@ -104,7 +109,7 @@ private:
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, smap_;
     PinBuf pin_desc_, pin_ctrl_;
     hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
     bool desc_pending_ = false;
@@ -115,6 +120,11 @@ private:
     hipError_t compact(size_t inst, uint32_t lo, uint32_t hi, uint32_t fill, const std::vector<uint4> &holes);
     std::vector<InstDesc> h_insts_;
     std::vector<TileDesc> h_tiles_;
+    std::vector<CellGroup> h_cells_;
+    // the batch of the last enqueue, redone with counted partitions when an
+    // S cell overflowed (force_counted_ set meanwhile)
+    std::vector<InstSpec> h_specs_;
+    bool force_counted_ = false;
     std::vector<ScanSeg> h_segs_;
     std::vector<ScanBlock> h_ad_, h_b_;
     std::vector<uint32_t> h_ctrl_;

@@ -199,6 +199,16 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Cursors of the S cell layout: group g's cursor of digit d starts at its
+// cell, cell0 + d * cap.  Grid (chunks of 256 digits, groups).
+__global__ __launch_bounds__(256) void k_cell_init(const CellGroup *__restrict__ cg, uint32_t *__restrict__ counts)
+{
+    const CellGroup g = cg[blockIdx.y];
+    const uint32_t F = 1u << g.bits;
+    for (uint32_t d = blockIdx.x * 256u + threadIdx.x; d < F; d += gridDim.x * 256u)
+        counts[g.cnt_off + d] = g.cell0 + d * g.cap;
+}
+
 // ---------------------------------------------------------------------------
 // k_hist: digit histogram of one tile (glb_partition_count.c:119-371),
 // added into the counters of the tile's group (the tiles one XCD scatters).
@@ -438,9 +448,13 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         const InstDesc &I = insts[td.inst];
         const uint32_t F = 1u << I.bits, mask = F - 1u;
         const bool heavy = (I.heavy != 0) && (td.rel == 1) && heavy_keys_of(I) != 0;
+        // R tiles of the cell layout record each heavy key's R rid (in the
+        // counted layout the histogram pass does); S tiles read the rids in
+        // a later launch
+        const bool rec = (I.heavy != 0) && (td.rel == 0) && (td.cap != 0) && heavy_keys_of(I) != 0;
         const uint32_t rn = min(ROUND, td.end - o);
         __syncthreads();   // the previous round's write-out and clearing are done
-        if (heavy && heavy_inst != td.inst) {
+        if ((heavy || rec) && heavy_inst != td.inst) {
             load_heavy(I, hk, hr);
             heavy_inst = td.inst;
             __syncthreads();
@@ -484,6 +498,17 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 }
             }
         } else {
+            if (rec) {
+                int hs[IPT];
+#pragma unroll
+                for (int j = 0; j < IPT; j++) {
+                    const uint32_t li = j * NT + threadIdx.x;
+                    hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
+                }
+#pragma unroll
+                for (int j = 0; j < IPT; j++)
+                    if (hs[j] >= 0) gst32(I.heavy_rids, hs[j], v[j].y);   // R keys are unique: one writer per slot
+            }
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
@@ -502,14 +527,15 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             const uint32_t hn = lst[F];
             if (hn) hres = atomicAdd(&out_hi[td.inst], hn);
         }
-        uint32_t cb[FMAX / NT];
+        uint32_t cb[FMAX / NT], cc[FMAX / NT];
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
             const uint32_t d = u * NT + threadIdx.x;
             cb[u] = 0;
+            cc[u] = 0;
             if (d < F) {
-                const uint32_t c = lst[d];
-                if (c) cb[u] = atomicAdd(&cur[d], c);
+                cc[u] = lst[d];
+                if (cc[u]) cb[u] = atomicAdd(&cur[d], cc[u]);
             }
         }
         const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
@@ -531,7 +557,16 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
             const uint32_t d = u * NT + threadIdx.x;
-            if (d < F) bs[d] = cb[u] - lst[d];
+            if (d < F) {
+                uint32_t b = cb[u];
+                if (td.cap != 0 && cc[u] != 0 && b + cc[u] > td.cell0 + (d + 1) * td.cap) {
+                    // the cell is full: the run goes to the sink and the
+                    // join is redone with counted partitions
+                    b = td.rel ? I.s_sink : I.r_sink;
+                    atomicOr(&err[td.inst], kErrCellOverflow);
+                }
+                bs[d] = b - lst[d];
+            }
         }
         if (heavy && threadIdx.x == 0) {
             // heavy pairs fill the output from its end downwards
@@ -684,11 +719,37 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
 // ---------------------------------------------------------------------------
 // k_make_items: the device work queue.  For every partition with tuples on
 // both sides, ceil(|S_p| / smax) items split its S range; all instances of a
-// batch append to one dense list.
+// batch append to one dense list.  With a cell layout the partition's tuples
+// of that relation are the fills of its kScatterGroups cells, which the
+// partition's SMap entry strings together; a cell's fill is clamped to its
+// capacity (an overflowed cell has been flagged, and the join is redone).
+// spart[inst] and rpart[inst] (zeroed before) receive the number of
+// partitioned S and R tuples.  Grid (instances, partition chunks of 1024).
 // ---------------------------------------------------------------------------
+// Partition p's SMap entry from its cells' fills (clamped to the capacity);
+// returns the partition's tuple count.
+__device__ __forceinline__ uint32_t cell_map(const CellGroup *cg, const uint32_t *counts, uint32_t p, SMap *out)
+{
+    SMap m;
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < kScatterGroups; g++) {
+        const uint32_t cap = cg[g].cap, cb = cg[g].cell0 + p * cap;
+        const uint32_t fill = cap ? min(counts[cg[g].cnt_off + p] - cb, cap) : 0u;
+        m.V[g] = n;
+        m.D[g] = cb - n;
+        n += fill;
+    }
+    *out = m;
+    return n;
+}
+
 __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
+                                                     const CellGroup *__restrict__ cells,
+                                                     const uint32_t *__restrict__ counts, SMap *__restrict__ smap,
                                                      uint32_t smax, WorkItem *__restrict__ items,
-                                                     uint32_t *__restrict__ nitems)
+                                                     uint32_t *__restrict__ nitems, uint32_t *__restrict__ spart,
+                                                     uint32_t *__restrict__ rpart)
 {
     __shared__ uint32_t wtmp[1024 / kWave + 1];
     __shared__ uint32_t cnt[1024];
@@ -697,12 +758,14 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
     const InstDesc &I = insts[inst];
     const uint32_t F = 1u << I.bits;
     const uint32_t *pr = pstart + I.pstartR, *ps = pstart + I.pstartS;
-    for (uint32_t p0 = 0; p0 < F; p0 += 1024) {
+    const uint32_t p0 = blockIdx.y * 1024u;
+    if (p0 >= F) return;   // whole block
+    {
         const uint32_t p = p0 + threadIdx.x;
         uint32_t nr = 0, ns = 0, c = 0;
         if (p < F) {
-            nr = pr[p + 1] - pr[p];
-            ns = ps[p + 1] - ps[p];
+            nr = I.r_cells ? cell_map(cells + (I.r_cells - 1), counts, p, smap + I.r_map + p) : pr[p + 1] - pr[p];
+            ns = I.s_cells ? cell_map(cells + (I.s_cells - 1), counts, p, smap + I.s_map + p) : ps[p + 1] - ps[p];
             c = (nr && ns) ? (ns + smax - 1) / smax : 0u;
         }
         cnt[threadIdx.x] = c;
@@ -715,14 +778,37 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
             WorkItem w;
             w.inst = inst;
             w.part = p;
-            w.sb = ps[p] + k * smax;
-            w.se = min(ps[p + 1], w.sb + smax);
-            w.rb = pr[p];
-            w.re = pr[p + 1];
-            w.pad[0] = w.pad[1] = 0;
+            if (I.s_cells) {
+                w.sb = k * smax;
+                w.se = min(ns, w.sb + smax);
+                w.smap = I.s_map + p + 1;
+            } else {
+                w.sb = ps[p] + k * smax;
+                w.se = min(ps[p + 1], w.sb + smax);
+                w.smap = 0;
+            }
+            if (I.r_cells) {
+                w.rb = 0;
+                w.re = nr;
+                w.rmap = I.r_map + p + 1;
+            } else {
+                w.rb = pr[p];
+                w.re = pr[p + 1];
+                w.rmap = 0;
+            }
             items[o + k] = w;
         }
         __syncthreads();
+        cnt[threadIdx.x] = ns;
+        __syncthreads();
+        const uint32_t all_s = block_exclusive_scan(cnt, 1024, wtmp);
+        cnt[threadIdx.x] = nr;
+        __syncthreads();
+        const uint32_t all_r = block_exclusive_scan(cnt, 1024, wtmp);
+        if (threadIdx.x == 0) {
+            if (all_s) atomicAdd(&spart[inst], all_s);
+            if (all_r) atomicAdd(&rpart[inst], all_r);
+        }
     }
 }
 
@@ -824,18 +910,54 @@ __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* powe
     }
 }
 
-// Load one probe round: S tuples [s0, s0 + rn) of the item, IPT per thread.
+// A partition's position map (SMap) in scalar registers: loaded through the
+// scalar cache (written by an earlier kernel, only read here).  Vector-typed
+// values, so that the selection stays in registers (an array here ends up in
+// scratch memory, indexed).
+typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
+struct SPos {
+    u32x8_t V, D;
+    __device__ __forceinline__ void load(const SMap *smap, uint32_t e)
+    {
+        if (e == 0) {
+            V = (u32x8_t)(0xFFFFFFFFu);
+            V[0] = 0;
+            D = (u32x8_t)(0u);
+            return;
+        }
+        typedef const __attribute__((address_space(4))) u32x8_t const_u32x8;
+        const_u32x8 *m = reinterpret_cast<const_u32x8 *>(reinterpret_cast<uintptr_t>(smap + (e - 1)));
+        V = m[0];
+        D = m[1];
+    }
+    // Sp index of partition-relative S position v
+    __device__ __forceinline__ uint32_t operator()(uint32_t v) const
+    {
+        uint32_t d = D[0];
+        d = (v >= V[1]) ? D[1] : d;
+        d = (v >= V[2]) ? D[2] : d;
+        d = (v >= V[3]) ? D[3] : d;
+        d = (v >= V[4]) ? D[4] : d;
+        d = (v >= V[5]) ? D[5] : d;
+        d = (v >= V[6]) ? D[6] : d;
+        d = (v >= V[7]) ? D[7] : d;
+        return v + d;
+    }
+};
+static_assert(kScatterGroups == 8, "SPos holds 8 cells");
+
+// Load one probe round: S positions [s0, s0 + rn) of the item, IPT per thread.
 template <int IPT>
-__device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, uint32_t s0, uint32_t se)
+__device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, const SPos &sp, uint32_t s0, uint32_t se)
 {
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         const uint32_t si = s0 + j * kJoinNT + threadIdx.x;
-        v[j] = (si < se) ? gld(S, si) : make_uint2(kEmptyKey, 0u);
+        v[j] = (si < se) ? gld(S, sp(si)) : make_uint2(kEmptyKey, 0u);
     }
 }
 
-__global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
+__global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts, const SMap *__restrict__ smap,
                                                   const WorkItem *__restrict__ items, uint32_t *__restrict__ ctrl,
                                                   uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill,
                                                   uint32_t *__restrict__ err, uint4 *__restrict__ holes,
@@ -849,7 +971,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
     if (threadIdx.x == 0) sh.item = atomicAdd(&ctrl[1], 1u);
     for (;;) {
         __syncthreads();
-        const uint32_t it = sh.item;
+        const uint32_t it = __builtin_amdgcn_readfirstlane(sh.item);
         if (it >= total_items) break;
         // the next item's index is taken now and published later in this
         // item (after the table scan on the hash path, else at the end), so
@@ -858,6 +980,9 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         if (threadIdx.x == 0) next_item = atomicAdd(&ctrl[1], 1u);
         const WorkItem w = items[it];
         const InstDesc &I = insts[w.inst];
+        SPos spos, rpos;
+        spos.load(smap, w.smap);
+        rpos.load(smap, w.rmap);
         const uint32_t rb = w.rb, re = w.re;
         const uint32_t nr = re - rb;
         uint32_t alg = I.alg;
@@ -881,11 +1006,11 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             for (uint32_t c0 = 0; c0 < nr; c0 += kNLChunk) {
                 const uint32_t cn = min(kNLChunk, nr - c0);
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = gld(I.Rp, rb + c0 + i); rk[i] = r.x; rv[i] = r.y; }
+                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); rk[i] = r.x; rv[i] = r.y; }
                 __syncthreads();
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
-                    load_round<IPT>(v, I.Sp, s0, w.se);
+                    load_round<IPT>(v, I.Sp, spos, s0, w.se);
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
                         bool hit = false;
@@ -916,7 +1041,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 while (np < cn) np <<= 1;
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < np; i += NT) {
-                    if (i < cn) { uint2 r = gld(I.Rp, rb + c0 + i); ra[i] = ((uint64_t)r.x << 32) | r.y; }
+                    if (i < cn) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); ra[i] = ((uint64_t)r.x << 32) | r.y; }
                     else ra[i] = ~0ull;
                 }
                 __syncthreads();
@@ -926,7 +1051,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     uint32_t sp = 1;
                     while (sp < sn) sp <<= 1;
                     for (uint32_t i = threadIdx.x; i < sp; i += NT) {
-                        if (i < sn) { uint2 sv = gld(I.Sp, s0 + i); sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
+                        if (i < sn) { uint2 sv = gld(I.Sp, spos(s0 + i)); sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
                         else sa[i] = ~0ull;
                     }
                     __syncthreads();
@@ -1013,7 +1138,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = rb + u * NT + threadIdx.x;
-                        if (i < re) r[u] = gld(I.Rp, i);
+                        if (i < re) r[u] = gld(I.Rp, rpos(i));
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++)
@@ -1024,7 +1149,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? gld(I.Rp, i) : make_uint2(kEmptyKey, 0u);
+                            c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
                         }
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
@@ -1068,7 +1193,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? gld(I.Rp, i) : make_uint2(kEmptyKey, 0u);
+                            c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
                         }
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
@@ -1108,13 +1233,13 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
-                load_round<IPT>(nv, I.Sp, w.sb, w.se);
+                load_round<IPT>(nv, I.Sp, spos, w.sb, w.se);
                 __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
-                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, s0 + ROUND, w.se);
+                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, spos, s0 + ROUND, w.se);
                     uint32_t hitm = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
@@ -1332,22 +1457,31 @@ hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint
 
 uint32_t cand_words() { return 2 * kCandSlots + 4; }
 
-hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
-                             WorkItem *items, uint32_t *nitems, hipStream_t st)
+hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st)
 {
-    if (!ninst) return hipSuccess;
-    hipLaunchKernelGGL(k_make_items, dim3(ninst), dim3(1024), 0, st, insts, pstart, smax, items, nitems);
+    if (!ncells) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_init, dim3((1u << kMaxBits) / 256u, ncells), dim3(256), 0, st, cells, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, uint32_t *ctrl,
+hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, const CellGroup *cells,
+                             const uint32_t *counts, SMap *smap, uint32_t smax, WorkItem *items, uint32_t *nitems,
+                             uint32_t *spart, uint32_t *rpart, hipStream_t st)
+{
+    if (!ninst) return hipSuccess;
+    hipLaunchKernelGGL(k_make_items, dim3(ninst, (1u << kMaxBits) / 1024u), dim3(1024), 0, st, insts, pstart, cells,
+                       counts, smap, smax, items, nitems, spart, rpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_join(const InstDesc *insts, const SMap *smap, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st)
 {
     size_t l = lds_join();
     hipError_t e = set_lds((const void *)k_join, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, pstart, items, ctrl, out_lo, out_fill, err,
+    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, smap, items, ctrl, out_lo, out_fill, err,
                        holes, max_holes);
     return hipGetLastError();
 }

@@ -21,7 +21,8 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
                        const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
                        uint32_t max_bits, hipStream_t st);
 // Persistent scatter over `grid` workgroups (rounded up to a multiple of 8);
-// counts holds the scanned group cursors and is advanced in place.
+// counts holds the group cursors (scanned, or cell bases) and is advanced in
+// place.
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
                           uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, uint32_t grid, hipStream_t st);
 // Sample blocks for heavy detection of an S of nS tuples (at most
@@ -34,9 +35,15 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 // cand_words() zeroed words at cand_base + cand_off.
 hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
                                uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
-hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, uint32_t smax,
-                             WorkItem *items, uint32_t *nitems, hipStream_t st);
-hipError_t launch_join(const InstDesc *insts, const uint32_t *pstart, const WorkItem *items, uint32_t *ctrl,
+// Cursors of the S cell layout: every group's cursor starts at its cell.
+hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st);
+// Work items of every instance; with the S cell layout also the partitions'
+// SMap entries.  spart[i] and rpart[i] receive instance i's partitioned S
+// and R tuple counts.
+hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, const CellGroup *cells,
+                             const uint32_t *counts, SMap *smap, uint32_t smax, WorkItem *items, uint32_t *nitems,
+                             uint32_t *spart, uint32_t *rpart, hipStream_t st);
+hipError_t launch_join(const InstDesc *insts, const SMap *smap, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st);
 hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st);

@@ -250,6 +250,7 @@ int spj_server_add_table(spj_server_t *sv, const spj_tuple_t *R, uint64_t nR, in
     s.nS = 0;
     s.out = dummy.as<uint2>();
     s.cfg = t->cfg;
+    s.r_contiguous = true;   // exported as one run per partition
     std::vector<uint32_t> counts, errs;
     if (e == hipSuccess) e = ctx.run({s}, counts, errs, nullptr);
     if (e == hipSuccess) e = t->rp.ensure(nR * 8 + 8);

@@ -104,6 +104,8 @@ int spj_tuner_get(uint64_t nR, uint64_t nS, double zipf, spj_config_t *cfg)
     cfg->scatter_ipt = (int32_t)r.ipt;
     cfg->tile_tuples = r.tile;
     cfg->items_s_max = r.smax;
+    cfg->part_layout = r.cells == 2 ? -1 : (int32_t)r.cells;
+    cfg->cell_capacity_pct = r.cell_pct ? (int32_t)r.cell_pct : -1;
     return SPJ_OK;
 }
 

@@ -39,6 +39,45 @@ def test_sort_merge_and_nested_loop(gpu, orc, alg):
     check_equal(sj, orc, R, S, pairs)
 
 
+@pytest.mark.parametrize("alg", [0, 2, 3])
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("heavy", [0, 1])
+def test_partition_layouts(gpu, orc, alg, layout, heavy):
+    """R and S partitioned counted (histogram pass) or into fixed-capacity
+    cells (no histogram pass), under every join algorithm, with skew, with
+    and without heavy keys (whose R rids the R cell scatter records)."""
+    sj = gpu
+    nR = {0: 1 << 17, 2: 1 << 15, 3: 2000}[alg]
+    R = sj.gen_r(nR, 70 + alg)
+    S = sj.gen_s(nR * 4, R, 1.1, 80 + alg)
+    print(f"case layout={layout} alg={alg} heavy={heavy} nR={nR}", flush=True)
+    pairs, t = sj.join(R, S, sj.config(algorithm=alg, part_layout=layout, heavy=heavy), timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    if layout == 0:
+        assert t["s_cells"] == 0 and t["r_cells"] == 0
+    if heavy:
+        assert t["heavy_keys"] > 0
+
+
+@pytest.mark.parametrize("pct", [10, 60, 100])
+def test_cell_overflow_redone_counted(gpu, orc, pct):
+    """Cells too small for their tuples: the scatter flags the overflow and
+    the join is redone with counted partitions, with the same result."""
+    sj = gpu
+    R = sj.gen_r(1 << 18, 91)
+    S = sj.gen_s(1 << 21, R, 0.6, 92)
+    print(f"case cell pct={pct}", flush=True)
+    pairs, t = sj.join(R, S, sj.config(part_layout=1, cell_capacity_pct=pct), timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    assert t["s_cells"] == 0 and t["r_cells"] == 0   # redone counted
+    pairs, t = sj.join(R, S, timed=True)   # automatic capacity: no overflow
+    check_equal(sj, orc, R, S, pairs)
+    assert t["s_cells"] == 1 and t["r_cells"] == 0   # R below 2^20 tuples: counted
+    pairs, t = sj.join(R, S, sj.config(part_layout=1), timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    assert t["s_cells"] == 1 and t["r_cells"] == 1
+
+
 def test_unmatched_and_zero_keys(gpu, orc):
     """S keys missing from R, key-0 tuples on both sides (skipped, as the
     reference treats key 0 as empty), heavy keys without an R partner."""
