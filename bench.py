@@ -77,8 +77,10 @@ def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
 def roofline(res):
     """Dominant kernel and its achieved algorithmic GB/s.
 
-    Algorithmic bytes per launch (one launch per phase per join):
-      k_hist    reads every tuple once:                 8 (|R| + |S|)
+    Algorithmic bytes per join (the phase's launches together: the scatter
+    is one launch, or two -- R, then S -- when R has cells and heavy keys):
+      k_hist    reads every tuple of a counted relation
+                once (cell relations skip it):          8 (|R_counted| + |S_counted|)
       k_scatter reads every tuple, writes it to its
                 partition (or a heavy pair to out):     16 (|R| + |S|)
       k_join    reads the partitioned R and S, writes
@@ -87,8 +89,9 @@ def roofline(res):
     p = res["phases"]
     nR, nS = res["nR"], res["nS"]
     rp, spn = p["r_partitioned"], p["s_partitioned"]
+    counted = (0 if p.get("r_cells") else nR) + (0 if p.get("s_cells") else nS)
     kern = {
-        "k_hist": (p["hist_ms"], 8 * (nR + nS)),
+        "k_hist": (p["hist_ms"], 8 * counted),
         "k_scatter": (p["scatter_ms"], 16 * (nR + nS)),
         "k_join": (p["join_ms"], 8 * (rp + spn) + 8 * spn),
     }

@@ -53,7 +53,8 @@ typedef struct {
     int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
-    int64_t tile_tuples;      /* tuples per partition-pass tile */
+    int64_t tile_tuples;      /* tuples per partition-pass tile of a counted relation (cells: two
+                                 scatter rounds, 2 * 1024 * scatter_ipt) */
     int64_t items_s_max;      /* S tuples per join work item */
     int32_t part_layout;      /* partitions of R and S: 0 counted (a histogram pass sizes them
                                  exactly), 1 cells (no histogram pass: fixed-capacity cells per

@@ -96,6 +96,29 @@ struct TileDesc {
     uint32_t cell0, cap;     // cell layout: the group's cells (cap = 0: counted layout)
 };
 
+// The tiles of one relation of one instance: ntiles tiles of `tile` tuples
+// (the last one shorter) at tile-list positions [first, first + ntiles),
+// in the scatter launch of tiles [launch0, launch0 + launch_n), whose
+// position decides a tile's scatter group.  The relation's group-0 counters
+// are at cnt_off0; cells = 1 + index of its first CellGroup (0: counted).
+// The device expands runs into TileDescs.
+struct TileRun {
+    uint32_t inst, rel;
+    uint32_t first, ntiles;
+    uint32_t tile, len;
+    uint64_t cnt_off0;
+    uint32_t cells, bits;
+    uint32_t launch0, launch_n;
+};
+
+// Scatter group of position p in a scatter launch of T tiles: XCD x walks
+// tiles [x * q + min(x, r), that + q + (x < r)) with q = T / 8, r = T % 8.
+constexpr uint32_t tile_group(uint32_t p, uint32_t T)
+{
+    return (p < (T & 7u) * ((T >> 3) + 1)) ? p / ((T >> 3) + 1)
+                                            : (T & 7u) + (p - (T & 7u) * ((T >> 3) + 1)) / (T >> 3);
+}
+
 // One (instance, relation) whose group counts are scanned together: t_count
 // groups (kScatterGroups), group g's counters at cnt_off0 + g * F; chunk
 // partials live at part_off; partition starts at pstart_off (F + 1 entries).

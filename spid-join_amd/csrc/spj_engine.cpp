@@ -145,7 +145,7 @@ Context::~Context()
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
     DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
-                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &smap_};
+                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &smap_, &tiles_};
     for (auto *b : bufs) b->release();
     pin_desc_.release();
     pin_ctrl_.release();
@@ -153,15 +153,16 @@ Context::~Context()
 }
 
 // ctrl layout (uint32): [0] nitems, [1] qhead, [2] holes recorded, [3] pad,
-// then n words each of: out_lo (pairs reserved by join items, from the front),
+// [4, 20) the tile queues of the two scatter launches, then n words each of: out_lo (pairs reserved by join items, from the front),
 // out_hi (heavy pairs, from the back), out_fill (pairs written by items),
 // err, nheavy, spart and rpart (S and R tuples in partitions).
 enum {
     kCtrlLo = 0, kCtrlHi = 1, kCtrlFill = 2, kCtrlErr = 3, kCtrlHeavy = 4, kCtrlSPart = 5, kCtrlRPart = 6,
     kCtrlArrays = 7
 };
-static inline size_t ctrl_words(size_t n) { return 4 + kCtrlArrays * n; }
-static inline size_t ctrl_at(size_t n, int which, size_t i) { return 4 + (size_t)which * n + i; }
+constexpr size_t kCtrlHead = 4 + 2 * kScatterGroups;
+static inline size_t ctrl_words(size_t n) { return kCtrlHead + kCtrlArrays * n; }
+static inline size_t ctrl_at(size_t n, int which, size_t i) { return kCtrlHead + (size_t)which * n + i; }
 
 // Capacity of one S cell that takes a group's tuples of one digit, from the
 // group's tuple count n and the fan-out F: by default 1.5 times the mean
@@ -181,7 +182,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     const std::vector<InstSpec> &specs = h_specs_;
     const size_t n = specs.size();
     h_insts_.assign(n, InstDesc{});
-    h_tiles_.clear();
     h_cells_.clear();
     h_segs_.clear();
     h_ad_.clear();
@@ -205,7 +205,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     std::vector<int> heavy_idx(n, -1);
     std::vector<char> cells[2] = {std::vector<char>(n, 0), std::vector<char>(n, 0)};
     std::vector<uint64_t> cnt0[2] = {std::vector<uint64_t>(n, 0), std::vector<uint64_t>(n, 0)};
-    std::vector<TileDesc> lists[4];   // R cells, R counted, S counted, S cells
+    std::vector<TileRun> lists[4];   // R cells, R counted, S counted, S cells
     bool split = false;
     for (size_t i = 0; i < n; i++) {
         const InstSpec &s = specs[i];
@@ -244,8 +244,13 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         items_bound += F + (uint64_t)s.nS / smax + 1;
         for (uint32_t rel = pre ? 1u : 0u; rel < 2; rel++) {
             const uint32_t len = rel ? s.nS : s.nR;
-            const uint32_t T = (len + c.tile - 1) / c.tile;
             const bool cl = cells[rel][i];
+            // cells: two scatter rounds per tile, handed out by the scatter's
+            // tile queues in small steps so the workgroups finish together;
+            // counted: larger tiles, each costing F counter atomics in the
+            // histogram pass
+            const uint32_t tile = cl ? 2u * kScatterNT * c.ipt : c.tile;
+            const uint32_t T = (len + tile - 1) / tile;
             cnt0[rel][i] = cnt_total;
             uint32_t seg_id = 0;
             if (!cl) {
@@ -260,15 +265,16 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
                 seg_id = (uint32_t)h_segs_.size();
                 h_segs_.push_back(sg);
             }
-            std::vector<TileDesc> &dst = lists[rel == 0 ? (cl ? 0 : 1) : (cl ? 3 : 2)];
-            for (uint32_t t = 0; t < T; t++) {
-                TileDesc td{};
-                td.inst = (uint32_t)i;
-                td.rel = rel;
-                td.begin = t * c.tile;
-                td.end = std::min<uint64_t>(len, (uint64_t)(t + 1) * c.tile);
-                td.cnt_off = cnt_total;   // + group * F below
-                dst.push_back(td);
+            if (T) {
+                TileRun ru{};
+                ru.inst = (uint32_t)i;
+                ru.rel = rel;
+                ru.ntiles = T;
+                ru.tile = tile;
+                ru.len = len;
+                ru.cnt_off0 = cnt_total;
+                ru.bits = c.bits;
+                lists[rel == 0 ? (cl ? 0 : 1) : (cl ? 3 : 2)].push_back(ru);
             }
             if (!cl) {
                 const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
@@ -281,32 +287,44 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             cnt_total += (uint64_t)kScatterGroups * F;
         }
     }
-    const uint32_t hist_first = (uint32_t)lists[0].size();
-    const uint32_t n_counted = (uint32_t)(lists[1].size() + lists[2].size());
-    for (auto &l : lists) h_tiles_.insert(h_tiles_.end(), l.begin(), l.end());
-    if (h_tiles_.size() > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    // scatter launches: [0, scat_split) and [scat_split, T)
-    const uint32_t scat_split = split ? (uint32_t)(lists[0].size() + lists[1].size()) : 0u;
-    // scatter groups: the tiles of XCD x's range (as the scatter kernel walks
-    // a launch's tiles) share group x's counters
-    std::vector<uint8_t> tgroup(h_tiles_.size());
-    auto assign_groups = [&](uint64_t t0, uint64_t t1) {
-        const uint64_t T = t1 - t0, q = T / kScatterGroups, r = T % kScatterGroups;
-        for (uint64_t k = 0; k < T; k++) {
-            const uint64_t g = (k < r * (q + 1)) ? k / (q + 1) : r + (k - r * (q + 1)) / q;
-            TileDesc &td = h_tiles_[t0 + k];
-            td.cnt_off += g * (uint64_t(1) << h_insts_[td.inst].bits);
-            tgroup[t0 + k] = (uint8_t)g;
+    // tile-list positions
+    uint64_t pos = 0, lsize[4] = {0, 0, 0, 0};
+    h_runs_.clear();
+    for (int l = 0; l < 4; l++)
+        for (TileRun &ru : lists[l]) {
+            ru.first = (uint32_t)pos;
+            pos += ru.ntiles;
+            lsize[l] += ru.ntiles;
         }
-    };
-    assign_groups(0, scat_split);
-    assign_groups(scat_split, h_tiles_.size());
-    // regions: contiguous, or cells sized by each group's share
-    std::vector<uint64_t> gsize((size_t)n * 2 * kScatterGroups, 0);
+    if (pos > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    ntiles_ = (uint32_t)pos;
+    const uint32_t hist_first = (uint32_t)lsize[0];
+    const uint32_t n_counted = (uint32_t)(lsize[1] + lsize[2]);
+    // scatter launches: [0, scat_split) and [scat_split, ntiles_)
+    const uint32_t scat_split = split ? (uint32_t)(lsize[0] + lsize[1]) : 0u;
+    for (auto &l : lists)
+        for (TileRun &ru : l) {
+            const bool second = ru.first >= scat_split;
+            ru.launch0 = second ? scat_split : 0u;
+            ru.launch_n = second ? ntiles_ - scat_split : scat_split;
+            h_runs_.push_back(ru);
+        }
+    // regions: contiguous, or cells sized by each group's share of the
+    // relation's tuples (group g's tiles: positions [gs, ge) of the launch)
     auto gk = [](size_t i, uint32_t rel, uint32_t g) { return (i * 2 + rel) * kScatterGroups + g; };
-    for (uint64_t t = 0; t < h_tiles_.size(); t++) {
-        const TileDesc &td = h_tiles_[t];
-        if (cells[td.rel][td.inst]) gsize[gk(td.inst, td.rel, tgroup[t])] += td.end - td.begin;
+    std::vector<uint64_t> gsize((size_t)n * 2 * kScatterGroups, 0);
+    for (const TileRun &ru : h_runs_) {
+        if (!cells[ru.rel][ru.inst]) continue;
+        const uint64_t T = ru.launch_n, q = T / kScatterGroups, r = T % kScatterGroups;
+        const uint64_t f = ru.first - ru.launch0, e = f + ru.ntiles;
+        for (uint32_t g = 0; g < kScatterGroups; g++) {
+            const uint64_t gs = g * q + std::min<uint64_t>(g, r), ge = gs + q + (g < r ? 1 : 0);
+            const uint64_t a = std::max(f, gs), b = std::min(e, ge);
+            if (a >= b) continue;
+            uint64_t tuples = (b - a) * ru.tile;
+            if (b == e) tuples -= (uint64_t)ru.ntiles * ru.tile - ru.len;   // the run's short last tile
+            gsize[gk(ru.inst, ru.rel, g)] += tuples;
+        }
     }
     std::vector<uint32_t> gcell0(gsize.size(), 0), gcap(gsize.size(), 0);
     for (size_t i = 0; i < n; i++) {
@@ -340,13 +358,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             total += region + kSinkTuples;
         }
     }
-    for (uint64_t t = 0; t < h_tiles_.size(); t++) {
-        TileDesc &td = h_tiles_[t];
-        if (!cells[td.rel][td.inst]) continue;
-        const size_t k = gk(td.inst, td.rel, tgroup[t]);
-        td.cell0 = gcell0[k];
-        td.cap = gcap[k];
-    }
+    for (TileRun &ru : h_runs_)
+        if (cells[ru.rel][ru.inst]) ru.cells = (ru.rel ? h_insts_[ru.inst].s_cells : h_insts_[ru.inst].r_cells);
 
     // ---- heavy-key jobs (sampling blocks of both phases)
     h_jobs_.clear();
@@ -370,8 +383,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // descriptor block layout (256-byte aligned arrays)
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t o_insts = 0;
-    const size_t o_tiles = o_insts + al(n * sizeof(InstDesc));
-    const size_t o_segs = o_tiles + al(h_tiles_.size() * sizeof(TileDesc));
+    const size_t o_runs = o_insts + al(n * sizeof(InstDesc));
+    const size_t o_segs = o_runs + al(h_runs_.size() * sizeof(TileRun));
     const size_t o_ad = o_segs + al(h_segs_.size() * sizeof(ScanSeg));
     const size_t o_b = o_ad + al(h_ad_.size() * sizeof(ScanBlock));
     const size_t o_jobs = o_b + al(h_b_.size() * sizeof(ScanBlock));
@@ -384,6 +397,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     SPJ_TRY(rp_.ensure(rp_total * 8));
     SPJ_TRY(sp_.ensure(sp_total * 8));
     SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
+    SPJ_TRY(tiles_.ensure((size_t)ntiles_ * sizeof(TileDesc)));
     SPJ_TRY(smap_.ensure(smap_total * sizeof(SMap)));
     SPJ_TRY(holes_.ensure(items_bound * sizeof(uint4)));
     max_holes_ = (uint32_t)std::min<uint64_t>(items_bound, 0xFFFFFFFFull);
@@ -411,7 +425,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     {
         char *h = pin_desc_.as<char>();
         if (n) memcpy(h + o_insts, h_insts_.data(), n * sizeof(InstDesc));
-        if (!h_tiles_.empty()) memcpy(h + o_tiles, h_tiles_.data(), h_tiles_.size() * sizeof(TileDesc));
+        if (!h_runs_.empty()) memcpy(h + o_runs, h_runs_.data(), h_runs_.size() * sizeof(TileRun));
         if (!h_segs_.empty()) memcpy(h + o_segs, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg));
         if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
         if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
@@ -422,7 +436,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     SPJ_TRY(hipEventRecord(desc_ev_, st));
     desc_pending_ = true;
     char *dd = desc_.as<char>();
-    const TileDesc *d_tiles = reinterpret_cast<const TileDesc *>(dd + o_tiles);
+    const TileRun *d_runs = reinterpret_cast<const TileRun *>(dd + o_runs);
+    const TileDesc *d_tiles = tiles_.as<TileDesc>();
     const ScanSeg *d_segs = reinterpret_cast<const ScanSeg *>(dd + o_segs);
     const ScanBlock *d_ad = reinterpret_cast<const ScanBlock *>(dd + o_ad);
     const ScanBlock *d_b = reinterpret_cast<const ScanBlock *>(dd + o_b);
@@ -453,7 +468,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 
     // ---- partition: histogram (counted tiles), scan, scatter
-    const uint32_t ntiles = (uint32_t)h_tiles_.size();
+    const uint32_t ntiles = ntiles_;
+    SPJ_TRY(launch_tiles(d_runs, (uint32_t)h_runs_.size(), d_cells, tiles_.as<TileDesc>(), ntiles, st));
     SPJ_TRY(launch_fill(counts_.as<uint32_t>(), cnt_total, 0u, st));
     SPJ_TRY(launch_cell_init(d_cells, (uint32_t)h_cells_.size(), counts_.as<uint32_t>(), st));
     SPJ_TRY(launch_hist(d_insts, d_tiles + hist_first, n_counted, counts_.as<uint32_t>(), max_bits_, st));
@@ -466,10 +482,10 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // when its tiles record heavy rids for the S tiles
     const uint32_t sgrid = (uint32_t)std::max(1, device_info().cus);
     if (scat_split)
-        SPJ_TRY(launch_scatter(d_insts, d_tiles, scat_split, counts_.as<uint32_t>(), out_hi, err, max_bits_, max_ipt_,
-                               sgrid, st));
+        SPJ_TRY(launch_scatter(d_insts, d_tiles, scat_split, counts_.as<uint32_t>(), out_hi, err, ctrl + 4, max_bits_,
+                               max_ipt_, sgrid, st));
     SPJ_TRY(launch_scatter(d_insts, d_tiles + scat_split, ntiles - scat_split, counts_.as<uint32_t>(), out_hi, err,
-                           max_bits_, max_ipt_, sgrid, st));
+                           ctrl + 4 + kScatterGroups, max_bits_, max_ipt_, sgrid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
     // ---- join

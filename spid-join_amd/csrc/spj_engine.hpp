@@ -109,7 +109,7 @@ private:
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, smap_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, smap_, tiles_;
     PinBuf pin_desc_, pin_ctrl_;
     hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
     bool desc_pending_ = false;
@@ -119,7 +119,8 @@ private:
     std::vector<uint32_t> h_ns_;
     hipError_t compact(size_t inst, uint32_t lo, uint32_t hi, uint32_t fill, const std::vector<uint4> &holes);
     std::vector<InstDesc> h_insts_;
-    std::vector<TileDesc> h_tiles_;
+    std::vector<TileRun> h_runs_;   // expanded into tiles_ on the device
+    uint32_t ntiles_ = 0;
     std::vector<CellGroup> h_cells_;
     // the batch of the last enqueue, redone with counted partitions when an
     // S cell overflowed (force_counted_ set meanwhile)

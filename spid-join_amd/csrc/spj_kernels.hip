@@ -199,7 +199,33 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Cursors of the S cell layout: group g's cursor of digit d starts at its
+// Expand tile runs (sorted by first, none empty) into one TileDesc per tile.
+__global__ __launch_bounds__(256) void k_tiles(const TileRun *__restrict__ runs, uint32_t nruns,
+                                               const CellGroup *__restrict__ cells, TileDesc *__restrict__ tiles,
+                                               uint32_t ntiles)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= ntiles) return;
+    uint32_t lo = 0, hi = nruns;   // the last run with first <= t
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (runs[m].first <= t) lo = m;
+        else hi = m;
+    }
+    const TileRun r = runs[lo];
+    const uint32_t g = tile_group(t - r.launch0, r.launch_n);
+    TileDesc td;
+    td.inst = r.inst;
+    td.rel = r.rel;
+    td.begin = (t - r.first) * r.tile;
+    td.end = (uint32_t)min((uint64_t)r.len, (uint64_t)td.begin + r.tile);
+    td.cnt_off = r.cnt_off0 + ((uint64_t)g << r.bits);
+    td.cell0 = r.cells ? cells[r.cells - 1 + g].cell0 : 0u;
+    td.cap = r.cells ? cells[r.cells - 1 + g].cap : 0u;
+    tiles[t] = td;
+}
+
+// Cursors of the cell layout: group g's cursor of digit d starts at its
 // cell, cell0 + d * cap.  Grid (chunks of 256 digits, groups).
 __global__ __launch_bounds__(256) void k_cell_init(const CellGroup *__restrict__ cg, uint32_t *__restrict__ counts)
 {
@@ -399,7 +425,8 @@ template <int IPT>
 __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restrict__ insts,
                                                         const TileDesc *__restrict__ tiles, uint32_t ntiles,
                                                         uint32_t max_bits, uint32_t *__restrict__ counts,
-                                                        uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err)
+                                                        uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err,
+                                                        uint32_t *__restrict__ tq)
 {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t NT = kScatterNT, ROUND = NT * IPT;
@@ -407,9 +434,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     constexpr uint32_t FMAX = 1u << kMaxBits;
     uint32_t ts, te;
     xcd_range(blockIdx.x & 7u, ntiles, ts, te);
-    const uint32_t W = gridDim.x >> 3;   // workgroups per XCD (grid is a multiple of 8)
-    uint32_t t = ts + (blockIdx.x >> 3);
-    if (t >= te) return;   // whole workgroup
+    uint32_t *const q = tq + (blockIdx.x & 7u);   // this XCD's tile queue (zeroed before the launch)
     const uint32_t FM = 1u << max_bits;
     // LDS layout
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
@@ -418,9 +443,24 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
     uint32_t *bs = lst;                                             // after staging: write base - start
     uint32_t *wtmp = lst + FM + 2;                                  // NT/64 + 1
-    uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base
+    uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base, [1] queue
 
+    // Tiles come from the XCD's queue, taken two ahead: t is being
+    // scattered, tnext follows it, misc[1] holds the one after (written by
+    // thread 0 a round after its atomic, read at the next tile switch).
+    if (threadIdx.x == 0) {
+        misc[0] = atomicAdd(q, 1u);
+        misc[1] = atomicAdd(q, 1u);
+        misc[2] = atomicAdd(q, 1u);
+    }
     for (uint32_t i = threadIdx.x; i < FM + 2; i += NT) lst[i] = 0;
+    __syncthreads();
+    uint32_t t = ts + misc[0], tnext = ts + misc[1];
+    if (t >= te) return;   // whole workgroup
+    __syncthreads();   // misc[0] and misc[1] read before reuse
+    if (threadIdx.x == 0) misc[1] = misc[2];
+    uint32_t qv = 0;
+    bool qpending = false;   // thread 0: a queue atomic whose value goes to misc[1]
     TileDesc td = tiles[t];
     uint32_t o = td.begin;
     uint32_t heavy_inst = DROP;   // instance whose heavy table is in LDS
@@ -433,7 +473,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         uint32_t tn = t, on = o + ROUND;
         TileDesc tdn = td;
         if (on >= td.end) {
-            tn = t + W;
+            tn = tnext;
             if (tn < te) {
                 tdn = tiles[tn];
                 on = tdn.begin;
@@ -553,6 +593,10 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // at the next round's start and does not wait there -- a wait that
         // would also cover this round's stores, issued after them.
         __builtin_amdgcn_s_waitcnt(0);
+        if (qpending) {   // thread 0; the atomic completed with the wait above
+            misc[1] = qv;
+            qpending = false;
+        }
         __syncthreads();   // staging has read the starts; lst[d] becomes digit d's base - start
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
@@ -579,6 +623,13 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         __syncthreads();   // the write-out has read the bases
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         if (!more) break;
+        if (tn != t) {   // next tile: the one after it was taken a round ago at least
+            tnext = ts + misc[1];
+            if (threadIdx.x == 0) {
+                qv = atomicAdd(q, 1u);
+                qpending = true;
+            }
+        }
         t = tn;
         td = tdn;
         o = on;
@@ -1401,7 +1452,8 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
 }
 
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
-                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, uint32_t grid, hipStream_t st)
+                          uint32_t *out_hi, uint32_t *err, uint32_t *tq, uint32_t max_bits, int ipt, uint32_t grid,
+                          hipStream_t st)
 {
     if (!ntiles) return hipSuccess;
     grid = (grid + 7u) & ~7u;   // whole XCD groups
@@ -1411,22 +1463,22 @@ hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t
         e = set_lds((const void *)k_scatter<12>, l);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<12>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
-                           out_hi, err);
+                           out_hi, err, tq);
     } else if (ipt == 10) {
         e = set_lds((const void *)k_scatter<10>, l);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<10>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
-                           out_hi, err);
+                           out_hi, err, tq);
     } else if (ipt == 8) {
         e = set_lds((const void *)k_scatter<8>, l);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<8>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
-                           out_hi, err);
+                           out_hi, err, tq);
     } else {
         e = set_lds((const void *)k_scatter<4>, l);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter<4>, dim3(grid), dim3(kScatterNT), l, st, insts, tiles, ntiles, max_bits, counts,
-                           out_hi, err);
+                           out_hi, err, tq);
     }
     return hipGetLastError();
 }
@@ -1456,6 +1508,14 @@ hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint
 }
 
 uint32_t cand_words() { return 2 * kCandSlots + 4; }
+
+hipError_t launch_tiles(const TileRun *runs, uint32_t nruns, const CellGroup *cells, TileDesc *tiles, uint32_t ntiles,
+                        hipStream_t st)
+{
+    if (!ntiles) return hipSuccess;
+    hipLaunchKernelGGL(k_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, st, runs, nruns, cells, tiles, ntiles);
+    return hipGetLastError();
+}
 
 hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st)
 {

@@ -22,9 +22,10 @@ hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk
                        uint32_t max_bits, hipStream_t st);
 // Persistent scatter over `grid` workgroups (rounded up to a multiple of 8);
 // counts holds the group cursors (scanned, or cell bases) and is advanced in
-// place.
+// place; tq holds 8 zeroed tile-queue counters (one per XCD).
 hipError_t launch_scatter(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
-                          uint32_t *out_hi, uint32_t *err, uint32_t max_bits, int ipt, uint32_t grid, hipStream_t st);
+                          uint32_t *out_hi, uint32_t *err, uint32_t *tq, uint32_t max_bits, int ipt, uint32_t grid,
+                          hipStream_t st);
 // Sample blocks for heavy detection of an S of nS tuples (at most
 // max_blocks, evenly strided); *stride receives the stride in blocks.
 uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride);
@@ -35,7 +36,10 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 // cand_words() zeroed words at cand_base + cand_off.
 hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
                                uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
-// Cursors of the S cell layout: every group's cursor starts at its cell.
+// The tile list from its runs (sorted by first, none empty).
+hipError_t launch_tiles(const TileRun *runs, uint32_t nruns, const CellGroup *cells, TileDesc *tiles, uint32_t ntiles,
+                        hipStream_t st);
+// Cursors of the cell layout: every group's cursor starts at its cell.
 hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st);
 // Work items of every instance; with the S cell layout also the partitions'
 // SMap entries.  spart[i] and rpart[i] receive instance i's partitioned S
