@@ -106,13 +106,14 @@ def roofline(res):
 
 
 def measured_traffic(kernel):
-    """Bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
-    plain-copy calibration) from the committed rocprofv3 counter summary of
-    this same workload (profiles/c2_traffic.json), or None."""
+    """HBM bytes of `kernel` per join (FETCH_SIZE + WRITE_SIZE over its
+    dispatches of one join, scaled by the plain-copy calibration) from the
+    committed rocprofv3 counter summary of this same workload
+    (profiles/c2_traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "c2_traffic.json")
     try:
         with open(path) as f:
-            per = json.load(f)["per_dispatch"]
+            per = json.load(f)["per_join"]
         # template instances carry their argument (k_scatter<12>)
         key = next(k for k in per if k == kernel or k.startswith(kernel + "<"))
         t = per[key]
