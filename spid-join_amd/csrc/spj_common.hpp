@@ -58,11 +58,11 @@ struct InstDesc {
     uint32_t table_log2;     // LDS hash table slots (log2) for this instance
     uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop
     // Cell layout of S (s_) and R (r_): 1 + index of the relation's first
-    // of kScatterGroups CellGroups (0 = counted layout), the region offset
-    // of a kSinkTuples sink for runs that overflow a cell, and the index of
-    // partition 0's SMap entry.
-    uint32_t s_cells, s_sink, s_map;
-    uint32_t r_cells, r_sink, r_map;
+    // of kScatterGroups CellGroups (0 = counted layout), and the region
+    // offset of a kSinkTuples sink for runs that overflow a cell.
+    uint32_t s_cells, s_sink;
+    uint32_t r_cells, r_sink;
+    uint32_t pad_[2];
 };
 
 // Cell layout of one scatter group of one relation: the group's cursors
@@ -159,16 +159,17 @@ constexpr uint32_t kPreSample = 16;
 // One join work item: partition `part` of instance `inst`, S range
 // [sb, se) and the partition's R range [rb, re), absolute positions in the
 // instance's partitioned regions (so the join reads one descriptor, not the
-// partition starts as well).  With a cell layout, the relation's range
-// holds partition-relative positions mapped through SMap entry smap - 1
-// (S) or rmap - 1 (R).
+// partition starts as well).  The S and R positions go through the item's
+// position maps (the identity for counted relations; with a cell layout the
+// ranges are partition-relative).  40 words, all the join reads of an item.
 struct WorkItem {
     uint32_t inst;
     uint32_t part;
     uint32_t sb, se;
     uint32_t rb, re;
-    uint32_t smap;           // 0: contiguous S
-    uint32_t rmap;           // 0: contiguous R
+    uint32_t pad[2];
+    SMap smap, rmap;
 };
+constexpr uint32_t kItemWords = sizeof(WorkItem) / 4;
 
 }  // namespace spj

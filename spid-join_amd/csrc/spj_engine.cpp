@@ -145,7 +145,7 @@ Context::~Context()
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
     DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
-                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &smap_, &tiles_};
+                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &tiles_};
     for (auto *b : bufs) b->release();
     pin_desc_.release();
     pin_ctrl_.release();
@@ -197,7 +197,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // has cells and heavy keys, R is scattered by a launch of its own (its
     // tiles record the heavy keys' R rids that the S scatter emits).
     uint64_t rp_total = 0, sp_total = 0, cnt_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
-    uint64_t smap_total = 0;
     // one S-range size per batch (the item kernel takes a single value)
     const uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
     uint32_t nheavy_inst = 0;
@@ -353,8 +352,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             }
             (rel ? d.s_cells : d.r_cells) = first;
             (rel ? d.s_sink : d.r_sink) = (uint32_t)region;
-            (rel ? d.s_map : d.r_map) = (uint32_t)smap_total;
-            smap_total += F;
             total += region + kSinkTuples;
         }
     }
@@ -398,7 +395,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     SPJ_TRY(sp_.ensure(sp_total * 8));
     SPJ_TRY(items_.ensure(items_bound * sizeof(WorkItem)));
     SPJ_TRY(tiles_.ensure((size_t)ntiles_ * sizeof(TileDesc)));
-    SPJ_TRY(smap_.ensure(smap_total * sizeof(SMap)));
     SPJ_TRY(holes_.ensure(items_bound * sizeof(uint4)));
     max_holes_ = (uint32_t)std::min<uint64_t>(items_bound, 0xFFFFFFFFull);
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
@@ -490,11 +486,11 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
 
     // ---- join
     SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), d_cells, counts_.as<uint32_t>(),
-                              smap_.as<SMap>(), smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
+                              smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
     const DeviceInfo &di = device_info();
     const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
-    SPJ_TRY(launch_join(d_insts, smap_.as<SMap>(), items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
+    SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
                         holes_.as<uint4>(), max_holes_, grid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
     specs_n_ = n;

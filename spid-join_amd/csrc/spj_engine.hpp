@@ -109,7 +109,7 @@ private:
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, smap_, tiles_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_;
     PinBuf pin_desc_, pin_ctrl_;
     hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
     bool desc_pending_ = false;

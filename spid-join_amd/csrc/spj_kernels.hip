@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_cell_init(const CellGroup *__restrict__
 __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ insts, const TileDesc *__restrict__ tiles,
                                               uint32_t ntiles, uint32_t *__restrict__ counts)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     const uint32_t t = xcd_tile(blockIdx.x, ntiles);
     const TileDesc td = tiles[t];
     const InstDesc &I = insts[td.inst];
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256) void k_scan_b(const ScanSeg *__restrict__ segs
 __global__ __launch_bounds__(1024) void k_scan_c(const ScanSeg *__restrict__ segs, uint32_t *__restrict__ pstart)
 {
     __shared__ uint32_t wtmp[1024 / kWave + 1];
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     const ScanSeg sg = segs[blockIdx.x];
     const uint32_t F = 1u << sg.bits;
     uint32_t *p = pstart + sg.pstart_off;
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                                                         uint32_t *__restrict__ out_hi, uint32_t *__restrict__ err,
                                                         uint32_t *__restrict__ tq)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     constexpr uint32_t NT = kScatterNT, ROUND = NT * IPT;
     constexpr uint32_t DROP = 0xFFFFFFFFu;
     constexpr uint32_t FMAX = 1u << kMaxBits;
@@ -656,7 +656,7 @@ constexpr uint32_t kCandSlots = 16384;
 __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                       uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     uint32_t *sk = lds;
     uint32_t *sc = lds + kSampleSlots;
     // this block's job: last job whose first_block[phase] <= blockIdx.x
@@ -777,11 +777,10 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
 // spart[inst] and rpart[inst] (zeroed before) receive the number of
 // partitioned S and R tuples.  Grid (instances, partition chunks of 1024).
 // ---------------------------------------------------------------------------
-// Partition p's SMap entry from its cells' fills (clamped to the capacity);
-// returns the partition's tuple count.
-__device__ __forceinline__ uint32_t cell_map(const CellGroup *cg, const uint32_t *counts, uint32_t p, SMap *out)
+// Partition p's position map from its cells' fills (clamped to the
+// capacity); returns the partition's tuple count.
+__device__ __forceinline__ uint32_t cell_map(const CellGroup *cg, const uint32_t *counts, uint32_t p, SMap &m)
 {
-    SMap m;
     uint32_t n = 0;
 #pragma unroll
     for (uint32_t g = 0; g < kScatterGroups; g++) {
@@ -791,13 +790,21 @@ __device__ __forceinline__ uint32_t cell_map(const CellGroup *cg, const uint32_t
         m.D[g] = cb - n;
         n += fill;
     }
-    *out = m;
     return n;
+}
+
+__device__ __forceinline__ void identity_map(SMap &m)
+{
+#pragma unroll
+    for (uint32_t g = 0; g < kScatterGroups; g++) {
+        m.V[g] = g ? 0xFFFFFFFFu : 0u;
+        m.D[g] = 0;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
                                                      const CellGroup *__restrict__ cells,
-                                                     const uint32_t *__restrict__ counts, SMap *__restrict__ smap,
+                                                     const uint32_t *__restrict__ counts,
                                                      uint32_t smax, WorkItem *__restrict__ items,
                                                      uint32_t *__restrict__ nitems, uint32_t *__restrict__ spart,
                                                      uint32_t *__restrict__ rpart)
@@ -814,9 +821,12 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
     {
         const uint32_t p = p0 + threadIdx.x;
         uint32_t nr = 0, ns = 0, c = 0;
+        SMap rm, sm;
+        identity_map(rm);
+        identity_map(sm);
         if (p < F) {
-            nr = I.r_cells ? cell_map(cells + (I.r_cells - 1), counts, p, smap + I.r_map + p) : pr[p + 1] - pr[p];
-            ns = I.s_cells ? cell_map(cells + (I.s_cells - 1), counts, p, smap + I.s_map + p) : ps[p + 1] - ps[p];
+            nr = I.r_cells ? cell_map(cells + (I.r_cells - 1), counts, p, rm) : pr[p + 1] - pr[p];
+            ns = I.s_cells ? cell_map(cells + (I.s_cells - 1), counts, p, sm) : ps[p + 1] - ps[p];
             c = (nr && ns) ? (ns + smax - 1) / smax : 0u;
         }
         cnt[threadIdx.x] = c;
@@ -832,21 +842,20 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
             if (I.s_cells) {
                 w.sb = k * smax;
                 w.se = min(ns, w.sb + smax);
-                w.smap = I.s_map + p + 1;
             } else {
                 w.sb = ps[p] + k * smax;
                 w.se = min(ps[p + 1], w.sb + smax);
-                w.smap = 0;
             }
             if (I.r_cells) {
                 w.rb = 0;
                 w.re = nr;
-                w.rmap = I.r_map + p + 1;
             } else {
                 w.rb = pr[p];
                 w.re = pr[p + 1];
-                w.rmap = 0;
             }
+            w.pad[0] = w.pad[1] = 0;
+            w.smap = sm;
+            w.rmap = rm;
             items[o + k] = w;
         }
         __syncthreads();
@@ -892,7 +901,8 @@ constexpr uint32_t kSortS = 2048;           // sort-merge: S chunk sorted per me
 constexpr uint32_t kNLChunk = 8192;
 
 struct JoinShared {
-    uint32_t item;
+    uint32_t item;   // the item to process next
+    uint32_t item2;  // the one after it (the queue runs two items ahead)
     uint32_t fill;   // pairs written in the item's reservation
     uint32_t base;   // reservation start
     uint32_t res;    // reservation length
@@ -961,26 +971,12 @@ __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* powe
     }
 }
 
-// A partition's position map (SMap) in scalar registers: loaded through the
-// scalar cache (written by an earlier kernel, only read here).  Vector-typed
-// values, so that the selection stays in registers (an array here ends up in
-// scratch memory, indexed).
+// A position map (SMap) in scalar registers.  Vector-typed values, so that
+// the selection stays in registers (an array here ends up in scratch memory,
+// indexed).
 typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
 struct SPos {
     u32x8_t V, D;
-    __device__ __forceinline__ void load(const SMap *smap, uint32_t e)
-    {
-        if (e == 0) {
-            V = (u32x8_t)(0xFFFFFFFFu);
-            V[0] = 0;
-            D = (u32x8_t)(0u);
-            return;
-        }
-        typedef const __attribute__((address_space(4))) u32x8_t const_u32x8;
-        const_u32x8 *m = reinterpret_cast<const_u32x8 *>(reinterpret_cast<uintptr_t>(smap + (e - 1)));
-        V = m[0];
-        D = m[1];
-    }
     // Sp index of partition-relative S position v
     __device__ __forceinline__ uint32_t operator()(uint32_t v) const
     {
@@ -1008,45 +1004,87 @@ __device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, const SPos 
     }
 }
 
-__global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts, const SMap *__restrict__ smap,
+// An item's 40 words in scalar registers: through the scalar cache, or from
+// a copy in LDS (uniform values, read by every lane).
+struct ItemRegs {
+    u32x8_t h;        // inst, part, sb, se, rb, re, -, -
+    SPos s, r;
+    __device__ __forceinline__ void load_global(const WorkItem *items, uint32_t it)
+    {
+        typedef const __attribute__((address_space(4))) u32x8_t const_u32x8;
+        const_u32x8 *m = reinterpret_cast<const_u32x8 *>(reinterpret_cast<uintptr_t>(items + it));
+        h = m[0];
+        s.V = m[1];
+        s.D = m[2];
+        r.V = m[3];
+        r.D = m[4];
+    }
+    static __device__ __forceinline__ u32x8_t lds8(const uint32_t *p)
+    {
+        const u32x8_t x = *reinterpret_cast<const u32x8_t *>(p);
+        u32x8_t y;
+#pragma unroll
+        for (int k = 0; k < 8; k++) y[k] = __builtin_amdgcn_readfirstlane(x[k]);
+        return y;
+    }
+    __device__ __forceinline__ void load_lds(const uint32_t *p)
+    {
+        h = lds8(p);
+        s.V = lds8(p + 8);
+        s.D = lds8(p + 16);
+        r.V = lds8(p + 24);
+        r.D = lds8(p + 32);
+    }
+};
+static_assert(kItemWords == 40 && kItemWords <= kWave, "ItemRegs holds 40 words, one wave copies them");
+
+__global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts,
                                                   const WorkItem *__restrict__ items, uint32_t *__restrict__ ctrl,
                                                   uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill,
                                                   uint32_t *__restrict__ err, uint4 *__restrict__ holes,
                                                   uint32_t max_holes)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     __shared__ JoinShared sh;
     __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
     constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
     const uint32_t total_items = ctrl[0];
-    if (threadIdx.x == 0) sh.item = atomicAdd(&ctrl[1], 1u);
+    // The queue runs two items ahead, so the next item's words can be
+    // loaded during the current item (at its probe's start, with the first
+    // round's tuples) and kept in LDS (nw, of item nw_item): the item start
+    // then reads LDS instead of waiting for a miss to HBM.
+    __shared__ __align__(32) uint32_t nw[kItemWords];
+    __shared__ uint32_t nw_item;
+    if (threadIdx.x == 0) {
+        sh.item = atomicAdd(&ctrl[1], 1u);
+        sh.item2 = atomicAdd(&ctrl[1], 1u);
+        nw_item = 0xFFFFFFFFu;
+    }
     for (;;) {
         __syncthreads();
         const uint32_t it = __builtin_amdgcn_readfirstlane(sh.item);
         if (it >= total_items) break;
-        // the next item's index is taken now and published later in this
-        // item (after the table scan on the hash path, else at the end), so
-        // the queue's round trip hides behind this item
-        uint32_t next_item = 0;
-        if (threadIdx.x == 0) next_item = atomicAdd(&ctrl[1], 1u);
-        const WorkItem w = items[it];
+        // The hash path takes the next item's index and this item's output
+        // reservation by atomics issued right before an explicit wait that
+        // it makes anyway (at its probe's start); no wait on their result
+        // registers lingers to the next item's start, where it would also
+        // cover (vector memory completes in order) this item's last pair
+        // stores.  The other paths take them at their start and end.
+        ItemRegs ir;
+        if (__builtin_amdgcn_readfirstlane(nw_item) == it) ir.load_lds(nw);
+        else ir.load_global(items, it);
+        struct { uint32_t inst, sb, se, rb, re; } w = {ir.h[0], ir.h[2], ir.h[3], ir.h[4], ir.h[5]};
+        const SPos &spos = ir.s, &rpos = ir.r;
         const InstDesc &I = insts[w.inst];
-        SPos spos, rpos;
-        spos.load(smap, w.smap);
-        rpos.load(smap, w.rmap);
         const uint32_t rb = w.rb, re = w.re;
         const uint32_t nr = re - rb;
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
         uint32_t *erri = err + w.inst;
-        // the item's output reservation; the hash path publishes the base
-        // after its table build, which hides the atomic's round trip
-        uint32_t res_base = 0;
         if (threadIdx.x == 0) {
             sh.res = w.se - w.sb;
             sh.fill = 0;
-            res_base = atomicAdd(&out_lo[w.inst], w.se - w.sb);
-            if (alg != 0) sh.base = res_base;
+            if (alg != 0) sh.base = atomicAdd(&out_lo[w.inst], w.se - w.sb);
         }
         __syncthreads();
 
@@ -1175,9 +1213,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
             const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
             constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
-            for (uint32_t q = 0; q < Q; q++) {
+            uint32_t q = 0;   // Q >= 1
+            do {
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < nb; i += NT) bcnt[i] = 0;
+                {
+                    uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
+                    asm volatile("" : "+v"(i0));
+                    for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
+                }
                 __syncthreads();
                 // R_p fits one table (Q == 1): hold it in registers (<= RPT
                 // tuples per thread) and read it once; otherwise each
@@ -1211,21 +1254,32 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 __syncthreads();
                 const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
-                if (q == 0 && threadIdx.x == 0) {
-                    // both atomics issued at the item's start have long
-                    // returned; publishing them here rather than at the
-                    // item's end keeps the end from waiting (vector memory
-                    // completes in order) for the probe's stores
-                    sh.base = res_base;   // read after the barriers below
-                    sh.item = next_item;  // read at the next item's start
-                }
                 if (ntab > kTableCap) {
                     if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
+                    uint32_t nx = 0, rbx = 0;   // the queue and reservation atomics, as below
+                    if (q == 0 && threadIdx.x == 0) {
+                        nx = atomicAdd(&ctrl[1], 1u);
+                        rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                    }
+                    __builtin_amdgcn_s_waitcnt(0);
+                    if (q == 0 && threadIdx.x == 0) {
+                        sh.item = sh.item2;
+                        sh.item2 = nx;
+                        sh.base = rbx;
+                    }
                     continue;   // uniform: ntab is the same in every thread
                 }
                 // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
                 // cursor of bucket b is bend[b + 1]
-                for (uint32_t i = threadIdx.x; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                {
+                    // the lane index is made opaque so that the addresses are
+                    // formed here: hoisted out of the item loop they spill,
+                    // and the reload waits (in order) for the previous item's
+                    // pair stores
+                    uint32_t i0 = threadIdx.x;
+                    asm volatile("" : "+v"(i0));
+                    for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                }
                 if (threadIdx.x == 0) bend[0] = 0;
                 __syncthreads();
                 // fill pass: a packed atomic advances one 16-bit cursor of a
@@ -1285,7 +1339,32 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
                 load_round<IPT>(nv, I.Sp, spos, w.sb, w.se);
+                uint32_t nx = 0, rbx = 0, pre = 0;
+                // the next item's words: wave 0, lane i loads word i
+                const uint32_t it2 = q == 0 && threadIdx.x < kItemWords ? sh.item2 : total_items;
+                if (it2 < total_items) {
+                    // the lane offset is made opaque so that the address is
+                    // formed here, not hoisted out of the item loop (a
+                    // 64-bit per-lane pointer there would spill)
+                    uint32_t lane = threadIdx.x;
+                    asm volatile("" : "+v"(lane));
+                    pre = gld32(reinterpret_cast<const uint32_t *>(items + it2), lane);
+                }
+                if (q == 0 && threadIdx.x == 0) {
+                    nx = atomicAdd(&ctrl[1], 1u);
+                    rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                }
                 __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
+                if (q == 0) {
+                    if (it2 < total_items) nw[threadIdx.x] = pre;
+                    if (threadIdx.x == 0) {
+                        if (it2 < total_items) nw_item = it2;
+                        sh.item = sh.item2;   // read at the next item's start
+                        sh.item2 = nx;
+                        sh.base = rbx;        // read by the pair emission
+                    }
+                    __syncthreads();
+                }
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
 #pragma unroll
@@ -1326,7 +1405,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     __builtin_amdgcn_s_waitcnt(0);
                     wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
                 }
-            }
+            } while (++q < Q);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1336,7 +1415,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 const uint32_t h = atomicAdd(&ctrl[2], 1u);
                 if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
             }
-            if (alg != 0) sh.item = next_item;   // the hash path published it after its scan
+            if (alg != 0) {   // the hash path advanced the queue at its probe's start
+                sh.item = sh.item2;
+                sh.item2 = atomicAdd(&ctrl[1], 1u);
+            }
         }
     }
 }
@@ -1525,23 +1607,23 @@ hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *c
 }
 
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, const CellGroup *cells,
-                             const uint32_t *counts, SMap *smap, uint32_t smax, WorkItem *items, uint32_t *nitems,
+                             const uint32_t *counts, uint32_t smax, WorkItem *items, uint32_t *nitems,
                              uint32_t *spart, uint32_t *rpart, hipStream_t st)
 {
     if (!ninst) return hipSuccess;
     hipLaunchKernelGGL(k_make_items, dim3(ninst, (1u << kMaxBits) / 1024u), dim3(1024), 0, st, insts, pstart, cells,
-                       counts, smap, smax, items, nitems, spart, rpart);
+                       counts, smax, items, nitems, spart, rpart);
     return hipGetLastError();
 }
 
-hipError_t launch_join(const InstDesc *insts, const SMap *smap, const WorkItem *items, uint32_t *ctrl,
+hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st)
 {
     size_t l = lds_join();
     hipError_t e = set_lds((const void *)k_join, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, smap, items, ctrl, out_lo, out_fill, err,
+    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
                        holes, max_holes);
     return hipGetLastError();
 }

@@ -41,13 +41,12 @@ hipError_t launch_tiles(const TileRun *runs, uint32_t nruns, const CellGroup *ce
                         hipStream_t st);
 // Cursors of the cell layout: every group's cursor starts at its cell.
 hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st);
-// Work items of every instance; with the S cell layout also the partitions'
-// SMap entries.  spart[i] and rpart[i] receive instance i's partitioned S
-// and R tuple counts.
+// Work items of every instance, with their partitions' position maps.
+// spart[i] and rpart[i] receive instance i's partitioned S and R tuple counts.
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, const CellGroup *cells,
-                             const uint32_t *counts, SMap *smap, uint32_t smax, WorkItem *items, uint32_t *nitems,
+                             const uint32_t *counts, uint32_t smax, WorkItem *items, uint32_t *nitems,
                              uint32_t *spart, uint32_t *rpart, hipStream_t st);
-hipError_t launch_join(const InstDesc *insts, const SMap *smap, const WorkItem *items, uint32_t *ctrl,
+hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st);
 hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st);
