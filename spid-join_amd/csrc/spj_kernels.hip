@@ -993,13 +993,15 @@ struct SPos {
 };
 static_assert(kScatterGroups == 8, "SPos holds 8 cells");
 
-// Load one probe round: S positions [s0, s0 + rn) of the item, IPT per thread.
+// Load one probe round: S positions [s0, s0 + rn) of the item, IPT per
+// thread (lane: the thread's index).
 template <int IPT>
-__device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, const SPos &sp, uint32_t s0, uint32_t se)
+__device__ __forceinline__ void load_round(uint2 *v, const uint2 *S, const SPos &sp, uint32_t s0, uint32_t se,
+                                           uint32_t lane)
 {
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-        const uint32_t si = s0 + j * kJoinNT + threadIdx.x;
+        const uint32_t si = s0 + j * kJoinNT + lane;
         v[j] = (si < se) ? gld(S, sp(si)) : make_uint2(kEmptyKey, 0u);
     }
 }
@@ -1099,7 +1101,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 __syncthreads();
                 for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
                     uint2 v[IPT];
-                    load_round<IPT>(v, I.Sp, spos, s0, w.se);
+                    load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
                         bool hit = false;
@@ -1338,7 +1340,15 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
                 // probe, with the next round's loads in flight
                 uint2 nv[IPT];
-                load_round<IPT>(nv, I.Sp, spos, w.sb, w.se);
+                {
+                    // the lane index is made opaque so that the first round's
+                    // positions are mapped here, not hoisted to the item's
+                    // start (where they would spill, and their reload wait,
+                    // in order, behind every store in flight)
+                    uint32_t lane = threadIdx.x;
+                    asm volatile("" : "+v"(lane));
+                    load_round<IPT>(nv, I.Sp, spos, w.sb, w.se, lane);
+                }
                 uint32_t nx = 0, rbx = 0, pre = 0;
                 // the next item's words: wave 0, lane i loads word i
                 const uint32_t it2 = q == 0 && threadIdx.x < kItemWords ? sh.item2 : total_items;
@@ -1369,7 +1379,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     uint2 v[IPT];
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
-                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, spos, s0 + ROUND, w.se);
+                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, spos, s0 + ROUND, w.se, threadIdx.x);
                     uint32_t hitm = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
