@@ -903,6 +903,7 @@ constexpr uint32_t kNLChunk = 8192;
 struct JoinShared {
     uint32_t item;   // the item to process next
     uint32_t item2;  // the one after it (the queue runs two items ahead)
+    uint32_t chunk;  // the probe's next free chunk
     uint32_t fill;   // pairs written in the item's reservation
     uint32_t base;   // reservation start
     uint32_t res;    // reservation length
@@ -992,6 +993,19 @@ struct SPos {
     }
 };
 static_assert(kScatterGroups == 8, "SPos holds 8 cells");
+
+// Load one probe chunk of a wave: S positions cb + j * 64 + lane (j < IPT)
+// below se.
+template <int IPT>
+__device__ __forceinline__ void load_chunk(uint2 *v, const uint2 *S, const SPos &sp, uint32_t cb, uint32_t se,
+                                           uint32_t lane)
+{
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t si = cb + j * kWave + lane;
+        v[j] = (si < se) ? gld(S, sp(si)) : make_uint2(kEmptyKey, 0u);
+    }
+}
 
 // Load one probe round: S positions [s0, s0 + rn) of the item, IPT per
 // thread (lane: the thread's index).
@@ -1311,6 +1325,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         }
                     }
                 }
+                if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
                 __syncthreads();   // bend[b + 1] = end of bucket b
                 // duplicate R keys (a2a_srj_build.c:265-271): each entry against
                 // the later entries of its bucket, in phases (keys, bucket ends,
@@ -1338,17 +1353,27 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     }
                     if (dup) atomicOr(erri, kErrDuplicateKey);
                 }
-                // probe, with the next round's loads in flight
+                // probe: the S range is cut into chunks of a wave's 64 x IPT
+                // tuples; wave i starts on chunk i and then takes chunks from
+                // a counter, so that waves that finish early take over the
+                // rest instead of waiting at the item's end for slow ones.
+                // The next chunk's loads are in flight while one is probed.
+                constexpr uint32_t CH = kWave * IPT;
+                const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
+                const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+                const uint32_t ln = lane_id();
                 uint2 nv[IPT];
                 {
-                    // the lane index is made opaque so that the first round's
+                    // the lane index is made opaque so that the first chunk's
                     // positions are mapped here, not hoisted to the item's
                     // start (where they would spill, and their reload wait,
                     // in order, behind every store in flight)
-                    uint32_t lane = threadIdx.x;
+                    uint32_t lane = ln;
                     asm volatile("" : "+v"(lane));
-                    load_round<IPT>(nv, I.Sp, spos, w.sb, w.se, lane);
+                    load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
                 }
+                uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
+                if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
                 uint32_t nx = 0, rbx = 0, pre = 0;
                 // the next item's words: wave 0, lane i loads word i
                 const uint32_t it2 = q == 0 && threadIdx.x < kItemWords ? sh.item2 : total_items;
@@ -1375,11 +1400,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     }
                     __syncthreads();
                 }
-                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                uint32_t cn = wv < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;   // this wave's next chunk
+                for (uint32_t c = wv; c < nch;) {
                     uint2 v[IPT];
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
-                    if (s0 + ROUND < w.se) load_round<IPT>(nv, I.Sp, spos, s0 + ROUND, w.se, threadIdx.x);
+                    if (cn < nch) load_chunk<IPT>(nv, I.Sp, spos, w.sb + cn * CH, w.se, ln);
+                    cg = 0;
+                    if (ln == 0 && cn < nch) cg = atomicAdd(&sh.chunk, 1u);   // the chunk after that
                     uint32_t hitm = 0;
 #pragma unroll
                     for (int j = 0; j < (int)IPT; j++) {
@@ -1409,10 +1437,12 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
                         hitm |= (hit ? 1u : 0u) << j;
                     }
-                    // wait here for the next round's loads (issued a probe
-                    // round ago), not at the next round's start, where the
-                    // wait would also cover the stores below
+                    // wait here for the next chunk's loads (issued a probe
+                    // ago), not at the next chunk's start, where the wait
+                    // would also cover the stores below
                     __builtin_amdgcn_s_waitcnt(0);
+                    c = cn;
+                    cn = cn < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;
                     wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
                 }
             } while (++q < Q);
