@@ -132,6 +132,20 @@ def test_radix_bits_sweep(gpu, orc, bits):
     check_equal(sj, orc, R, S, pairs)
 
 
+@pytest.mark.parametrize("copies", [2, 3, 5])
+def test_duplicate_r_key_in_large_partition(gpu, copies):
+    """A key repeated `copies` times among 2^18 R tuples (partitions of
+    thousands of tuples, its bucket holding at least `copies` entries) is
+    reported as a duplicate."""
+    sj = gpu
+    R = sj.gen_r(1 << 18, 11 + copies)
+    R["key"][1000:1000 + copies] = R["key"][77]
+    S = sj.gen_s(1 << 19, sj.gen_r(1 << 18, 11 + copies), 0.0, 12)
+    with pytest.raises(sj.SpjError) as e:
+        sj.join(R, S)
+    assert e.value.code == -4
+
+
 def test_batch_matches_single(gpu, orc):
     sj = gpu
     rng = np.random.default_rng(9)
