@@ -111,13 +111,10 @@ struct TileRun {
     uint32_t launch0, launch_n;
 };
 
-// Scatter group of position p in a scatter launch of T tiles: XCD x walks
-// tiles [x * q + min(x, r), that + q + (x < r)) with q = T / 8, r = T % 8.
-constexpr uint32_t tile_group(uint32_t p, uint32_t T)
-{
-    return (p < (T & 7u) * ((T >> 3) + 1)) ? p / ((T >> 3) + 1)
-                                            : (T & 7u) + (p - (T & 7u) * ((T >> 3) + 1)) / (T >> 3);
-}
+// Scatter group of position p in a scatter launch: XCD x takes tiles x,
+// x + 8, x + 16, ... (dealt round-robin, so that every XCD gets the same
+// mix of relations and instances and the XCDs finish together).
+constexpr uint32_t tile_group(uint32_t p) { return p % kScatterGroups; }
 
 // One (instance, relation) whose group counts are scanned together: t_count
 // groups (kScatterGroups), group g's counters at cnt_off0 + g * F; chunk

@@ -309,19 +309,17 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             h_runs_.push_back(ru);
         }
     // regions: contiguous, or cells sized by each group's share of the
-    // relation's tuples (group g's tiles: positions [gs, ge) of the launch)
+    // relation's tuples
     auto gk = [](size_t i, uint32_t rel, uint32_t g) { return (i * 2 + rel) * kScatterGroups + g; };
     std::vector<uint64_t> gsize((size_t)n * 2 * kScatterGroups, 0);
     for (const TileRun &ru : h_runs_) {
         if (!cells[ru.rel][ru.inst]) continue;
-        const uint64_t T = ru.launch_n, q = T / kScatterGroups, r = T % kScatterGroups;
-        const uint64_t f = ru.first - ru.launch0, e = f + ru.ntiles;
+        const uint64_t f = ru.first - ru.launch0, e = f + ru.ntiles;   // positions in the launch
         for (uint32_t g = 0; g < kScatterGroups; g++) {
-            const uint64_t gs = g * q + std::min<uint64_t>(g, r), ge = gs + q + (g < r ? 1 : 0);
-            const uint64_t a = std::max(f, gs), b = std::min(e, ge);
-            if (a >= b) continue;
-            uint64_t tuples = (b - a) * ru.tile;
-            if (b == e) tuples -= (uint64_t)ru.ntiles * ru.tile - ru.len;   // the run's short last tile
+            // positions p in [f, e) with tile_group(p) == g
+            auto below = [g](uint64_t x) { return (x + kScatterGroups - 1 - g) / kScatterGroups; };
+            uint64_t tuples = (below(e) - below(f)) * ru.tile;
+            if (tile_group((uint32_t)(e - 1)) == g) tuples -= (uint64_t)ru.ntiles * ru.tile - ru.len;   // short last tile
             gsize[gk(ru.inst, ru.rel, g)] += tuples;
         }
     }

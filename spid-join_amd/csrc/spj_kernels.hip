@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_tiles(const TileRun *__restrict__ runs,
         else hi = m;
     }
     const TileRun r = runs[lo];
-    const uint32_t g = tile_group(t - r.launch0, r.launch_n);
+    const uint32_t g = tile_group(t - r.launch0);
     TileDesc td;
     td.inst = r.inst;
     td.rel = r.rel;
@@ -371,22 +371,16 @@ __global__ __launch_bounds__(256) void k_scan_d(const ScanSeg *__restrict__ segs
 // Output space comes from cursors, one per (instance, relation, group,
 // digit): the scan leaves the start offset of group g's run of digit d in
 // counts[cnt_off + d], and each round reserves its run with one atomicAdd
-// per digit.  Group g is the tiles of XCD g's range, and block b runs on
-// XCD b % 8 and walks that range, so all workgroups of one XCD share the
-// cursors: their runs of a digit are adjacent, and the lines they write
-// partly fill up in that XCD's L2 instead of leaving half-written.  The
+// per digit.  Group g is tiles g, g + 8, g + 16, ... of the launch, and
+// block b runs on XCD b % 8 and takes that group's tiles from a queue, so
+// all workgroups of one XCD share the cursors: their runs of a digit are
+// adjacent, and the lines they write partly fill up in that XCD's L2
+// instead of leaving half-written.  The
 // order of tuples inside a partition therefore depends on timing; the
 // partition contents do not.
 // For S with a heavy table, heavy tuples form bucket F and are emitted
 // straight to the output as (R rid, S rid).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void xcd_range(uint32_t x, uint32_t T, uint32_t &ts, uint32_t &te)
-{
-    const uint32_t q = T >> 3, r = T & 7u;
-    ts = x * q + min(x, r);
-    te = ts + q + (x < r ? 1u : 0u);
-}
-
 template <int IPT>
 __device__ __forceinline__ void load_tile_round(uint2 *v, const InstDesc *__restrict__ insts, const TileDesc &td,
                                                 uint32_t o)
@@ -432,9 +426,9 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     constexpr uint32_t NT = kScatterNT, ROUND = NT * IPT;
     constexpr uint32_t DROP = 0xFFFFFFFFu;
     constexpr uint32_t FMAX = 1u << kMaxBits;
-    uint32_t ts, te;
-    xcd_range(blockIdx.x & 7u, ntiles, ts, te);
-    uint32_t *const q = tq + (blockIdx.x & 7u);   // this XCD's tile queue (zeroed before the launch)
+    const uint32_t x = blockIdx.x & 7u;           // this XCD's group: tiles x + 8 k
+    uint32_t *const q = tq + x;                     // its tile queue (k; zeroed before the launch)
+    const uint32_t te = ntiles;
     const uint32_t FM = 1u << max_bits;
     // LDS layout
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
@@ -455,7 +449,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     }
     for (uint32_t i = threadIdx.x; i < FM + 2; i += NT) lst[i] = 0;
     __syncthreads();
-    uint32_t t = ts + misc[0], tnext = ts + misc[1];
+    uint32_t t = x + kScatterGroups * misc[0], tnext = x + kScatterGroups * misc[1];
     if (t >= te) return;   // whole workgroup
     __syncthreads();   // misc[0] and misc[1] read before reuse
     if (threadIdx.x == 0) misc[1] = misc[2];
@@ -624,7 +618,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
         if (!more) break;
         if (tn != t) {   // next tile: the one after it was taken a round ago at least
-            tnext = ts + misc[1];
+            tnext = x + kScatterGroups * misc[1];
             if (threadIdx.x == 0) {
                 qv = atomicAdd(q, 1u);
                 qpending = true;
