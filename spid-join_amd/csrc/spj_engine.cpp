@@ -215,13 +215,17 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
         const bool pre = s.Rp_pre != nullptr;
         // cells: no histogram pass.  Cursors are 32-bit, so a relation's
-        // cell region plus all its tuples stays below 2^32.
-        auto use_cells = [&](uint32_t len) {
+        // cell region plus all its tuples stays below 2^32.  Automatically
+        // for large hash-join relations; for S only with heavy-key handling,
+        // which takes the keys frequent enough to overflow a cell (R keys
+        // are unique).
+        const bool heavy_on = c.heavy && s.nS > 0 && !pre;
+        auto use_cells = [&](uint32_t len, bool skew_safe) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
-                   (c.cells == 1 || (c.cells == 2 && c.alg == 0 && len >= (1u << 20)));
+                   (c.cells == 1 || (c.cells == 2 && c.alg == 0 && len >= (1u << 20) && skew_safe));
         };
-        cells[0][i] = !pre && !s.r_contiguous && use_cells(s.nR);
-        cells[1][i] = use_cells(s.nS);
+        cells[0][i] = !pre && !s.r_contiguous && use_cells(s.nR, true);
+        cells[1][i] = use_cells(s.nS, heavy_on);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
@@ -229,7 +233,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         d.nR = s.nR;
         d.nS = s.nS;
         d.bits = c.bits;
-        d.heavy = (c.heavy && s.nS > 0 && !pre) ? 1u : 0u;
+        d.heavy = heavy_on ? 1u : 0u;
         d.table_log2 = c.table_log2;
         d.alg = c.alg;
         d.pstartR = pst_total;

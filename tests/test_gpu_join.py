@@ -73,6 +73,9 @@ def test_cell_overflow_redone_counted(gpu, orc, pct):
     pairs, t = sj.join(R, S, timed=True)   # automatic capacity: no overflow
     check_equal(sj, orc, R, S, pairs)
     assert t["s_cells"] == 1 and t["r_cells"] == 0   # R below 2^20 tuples: counted
+    pairs, t = sj.join(R, S, sj.config(heavy=0), timed=True)   # no heavy handling: S stays counted
+    check_equal(sj, orc, R, S, pairs)
+    assert t["s_cells"] == 0
     pairs, t = sj.join(R, S, sj.config(part_layout=1), timed=True)
     check_equal(sj, orc, R, S, pairs)
     assert t["s_cells"] == 1 and t["r_cells"] == 1
