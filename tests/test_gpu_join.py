@@ -187,6 +187,43 @@ def test_batch_many_tiny_and_empty(gpu, orc):
         check_equal(sj, orc, R, S, p)
 
 
+def test_batch_cell_overflow_redone(gpu, orc):
+    """A batch whose instances all use cells too small for them: the batch is
+    redone with counted partitions and every instance's result is exact."""
+    sj = gpu
+    rng = np.random.default_rng(19)
+    Rs, Ss = [], []
+    for i in range(12):
+        nR = int(2 ** rng.uniform(12, 17))
+        R = sj.gen_r(nR, 600 + i)
+        Rs.append(R)
+        Ss.append(sj.gen_s(nR * int(rng.choice([1, 4])), R, float(rng.uniform(0, 1.2)), 700 + i))
+    print("case batch cell overflow", flush=True)
+    outs, t = sj.join_batch(Rs, Ss, sj.config(part_layout=1, cell_capacity_pct=60), timed=True)
+    for R, S, p in zip(Rs, Ss, outs):
+        check_equal(sj, orc, R, S, p)
+    assert t["s_cells"] == 0
+
+
+def test_join_instance_lifecycle_cells(gpu, orc):
+    """The asynchronous JoinInstance path on sizes that use the cell layout
+    with heavy keys (R scattered first, recording their rids)."""
+    sj = gpu
+    R = sj.gen_r(1 << 20, 43)
+    S = sj.gen_s(1 << 22, R, 1.1, 44)
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    d = sj.JoinInstance.design(join_alg=0, bank_set=8)
+    ji.prepare(d)
+    ji.start(d)
+    pairs = ji.finish()
+    check_equal(sj, orc, R, S, pairs)
+    t = ji.phase_times()
+    assert t["s_cells"] == 1 and t["r_cells"] == 1 and t["heavy_keys"] > 0
+    ji.close()
+
+
 def test_join_instance_lifecycle(gpu, orc):
     """LoadColumn / PrepareJoin / StartJoin (async) / FinishJoin, query.cc."""
     sj = gpu
