@@ -116,6 +116,26 @@ struct TileRun {
 // mix of relations and instances and the XCDs finish together).
 constexpr uint32_t tile_group(uint32_t p) { return p % kScatterGroups; }
 
+// Everything k_prep does for a batch (see there): words to zero, the tile
+// runs to expand, the cell groups whose cursors to set.
+struct PrepArgs {
+    uint32_t *ctrl;
+    uint64_t nctrl;
+    uint32_t *heavy;
+    uint64_t nheavy;
+    uint32_t *cand;
+    uint64_t ncand;
+    uint32_t *counts;
+    uint64_t nzero;           // counters of counted relations: counts[0, nzero)
+    const TileRun *runs;
+    uint32_t nruns;
+    uint32_t ntiles;
+    TileDesc *tiles;
+    const CellGroup *cells;
+    uint32_t ncells;
+    uint32_t pad;
+};
+
 // One (instance, relation) whose group counts are scanned together: t_count
 // groups (kScatterGroups), group g's counters at cnt_off0 + g * F; chunk
 // partials live at part_off; partition starts at pstart_off (F + 1 entries).

@@ -196,7 +196,10 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // histogram pass reads the counted ones (a contiguous range); when R
     // has cells and heavy keys, R is scattered by a launch of its own (its
     // tiles record the heavy keys' R rids that the S scatter emits).
-    uint64_t rp_total = 0, sp_total = 0, cnt_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
+    // counters: the counted relations' first (zeroed), then the cell
+    // relations' (set to the cell bases), so the two never overlap
+    uint64_t cnt_counted = 0, cnt_cells = 0;
+    uint64_t rp_total = 0, sp_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
     // one S-range size per batch (the item kernel takes a single value)
     const uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
     uint32_t nheavy_inst = 0;
@@ -254,6 +257,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             // histogram pass
             const uint32_t tile = cl ? 2u * kScatterNT * c.ipt : c.tile;
             const uint32_t T = (len + tile - 1) / tile;
+            uint64_t &cnt_total = cl ? cnt_cells : cnt_counted;   // cell offsets are shifted below
             cnt0[rel][i] = cnt_total;
             uint32_t seg_id = 0;
             if (!cl) {
@@ -290,6 +294,12 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             cnt_total += (uint64_t)kScatterGroups * F;
         }
     }
+    const uint64_t cnt_total = cnt_counted + cnt_cells;
+    for (size_t i = 0; i < n; i++)
+        for (uint32_t rel = 0; rel < 2; rel++)
+            if (cells[rel][i]) cnt0[rel][i] += cnt_counted;
+    for (int l : {0, 3})
+        for (TileRun &ru : lists[l]) ru.cnt_off0 += cnt_counted;
     // tile-list positions
     uint64_t pos = 0, lsize[4] = {0, 0, 0, 0};
     h_runs_.clear();
@@ -453,23 +463,36 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     uint32_t *nheavy = ctrl + ctrl_at(n, kCtrlHeavy, 0), *spart = ctrl + ctrl_at(n, kCtrlSPart, 0);
     uint32_t *rpart = ctrl + ctrl_at(n, kCtrlRPart, 0);
     if (timed) SPJ_TRY(hipEventRecord(ev_[0], st));
-    SPJ_TRY(launch_fill(ctrl, ctrl_words(n), 0u, st));
+    // ---- one preparation launch: control words, heavy tables and candidate
+    // maps, counters, tiles, cell cursors
+    const uint32_t ntiles = ntiles_;
+    {
+        PrepArgs pa{};
+        pa.ctrl = ctrl;
+        pa.nctrl = ctrl_words(n);
+        pa.heavy = heavy_.as<uint32_t>();
+        pa.nheavy = (uint64_t)nheavy_inst * 2 * kHeavySlots;
+        pa.cand = cand_.as<uint32_t>();
+        pa.ncand = (uint64_t)nheavy_inst * cand_words();
+        pa.counts = counts_.as<uint32_t>();
+        pa.nzero = cnt_counted;
+        pa.runs = d_runs;
+        pa.nruns = (uint32_t)h_runs_.size();
+        pa.ntiles = ntiles;
+        pa.tiles = tiles_.as<TileDesc>();
+        pa.cells = d_cells;
+        pa.ncells = (uint32_t)h_cells_.size();
+        SPJ_TRY(launch_prep(pa, st));
+    }
 
     // ---- heavy keys: two sampling launches (pre-sample, then the rest when
     // the pre-sample saw a candidate) and one selection launch for all
-    if (nheavy_inst) {
-        SPJ_TRY(launch_fill(heavy_.as<uint32_t>(), (uint64_t)nheavy_inst * 2 * kHeavySlots, 0u, st));
-        SPJ_TRY(launch_fill(cand_.as<uint32_t>(), (uint64_t)nheavy_inst * cand_words(), 0u, st));
+    if (nheavy_inst)
         SPJ_TRY(launch_heavy_detect(d_insts, d_jobs, (uint32_t)h_jobs_.size(), hblocks, cand_.as<uint32_t>(), nheavy,
                                     st));
-    }
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 
     // ---- partition: histogram (counted tiles), scan, scatter
-    const uint32_t ntiles = ntiles_;
-    SPJ_TRY(launch_tiles(d_runs, (uint32_t)h_runs_.size(), d_cells, tiles_.as<TileDesc>(), ntiles, st));
-    SPJ_TRY(launch_fill(counts_.as<uint32_t>(), cnt_total, 0u, st));
-    SPJ_TRY(launch_cell_init(d_cells, (uint32_t)h_cells_.size(), counts_.as<uint32_t>(), st));
     SPJ_TRY(launch_hist(d_insts, d_tiles + hist_first, n_counted, counts_.as<uint32_t>(), max_bits_, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
     SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), d_ad, (uint32_t)h_ad_.size(), d_b, (uint32_t)h_b_.size(),

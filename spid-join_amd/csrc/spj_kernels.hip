@@ -191,21 +191,11 @@ __device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key)
     return s;
 }
 
-// ---------------------------------------------------------------------------
-// k_fill: small fill kernel (graph-capture safe alternative to memset).
-// ---------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+// Tile t of the list from its run (runs sorted by first, none empty).
+__device__ __forceinline__ void expand_tile(const TileRun *__restrict__ runs, uint32_t nruns,
+                                            const CellGroup *__restrict__ cells, TileDesc *__restrict__ tiles,
+                                            uint32_t t)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
-}
-
-// Expand tile runs (sorted by first, none empty) into one TileDesc per tile.
-__global__ __launch_bounds__(256) void k_tiles(const TileRun *__restrict__ runs, uint32_t nruns,
-                                               const CellGroup *__restrict__ cells, TileDesc *__restrict__ tiles,
-                                               uint32_t ntiles)
-{
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= ntiles) return;
     uint32_t lo = 0, hi = nruns;   // the last run with first <= t
     while (hi - lo > 1) {
         const uint32_t m = (lo + hi) >> 1;
@@ -225,14 +215,32 @@ __global__ __launch_bounds__(256) void k_tiles(const TileRun *__restrict__ runs,
     tiles[t] = td;
 }
 
-// Cursors of the cell layout: group g's cursor of digit d starts at its
-// cell, cell0 + d * cap.  Grid (chunks of 256 digits, groups).
-__global__ __launch_bounds__(256) void k_cell_init(const CellGroup *__restrict__ cg, uint32_t *__restrict__ counts)
+// k_prep: everything a batch needs before its first pass, in one launch
+// (each of these was a launch of a few microseconds plus a gap):
+//   y = 0: zero the control words, the heavy tables, the candidate maps and
+//          the counted relations' digit counters; expand the tile runs;
+//   y = 1: the cell layout's cursors, group g's cursor of digit d at its
+//          cell, cell0 + d * cap (one block per cell group).
+// The counted relations' counters come before the cell relations' in the
+// counter array, so the two halves never write the same words.
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a)
 {
-    const CellGroup g = cg[blockIdx.y];
-    const uint32_t F = 1u << g.bits;
-    for (uint32_t d = blockIdx.x * 256u + threadIdx.x; d < F; d += gridDim.x * 256u)
-        counts[g.cnt_off + d] = g.cell0 + d * g.cap;
+    if (blockIdx.y == 0) {
+        const uint64_t n0 = a.nctrl, n1 = n0 + a.nheavy, n2 = n1 + a.ncand, n3 = n2 + a.nzero, n4 = n3 + a.ntiles;
+        for (uint64_t w = (uint64_t)blockIdx.x * 256u + threadIdx.x; w < n4; w += (uint64_t)gridDim.x * 256u) {
+            if (w < n0) a.ctrl[w] = 0;
+            else if (w < n1) a.heavy[w - n0] = 0;
+            else if (w < n2) a.cand[w - n1] = 0;
+            else if (w < n3) a.counts[w - n2] = 0;
+            else expand_tile(a.runs, a.nruns, a.cells, a.tiles, (uint32_t)(w - n3));
+        }
+    } else {
+        for (uint32_t c = blockIdx.x; c < a.ncells; c += gridDim.x) {
+            const CellGroup g = a.cells[c];
+            const uint32_t F = 1u << g.bits;
+            for (uint32_t d = threadIdx.x; d < F; d += 256u) a.counts[g.cnt_off + d] = g.cell0 + d * g.cap;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1532,15 +1540,6 @@ size_t lds_join()
 
 uint32_t join_table_cap() { return kTableCap; }
 
-hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
-{
-    if (!n) return hipSuccess;
-    uint64_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, p, n, v);
-    return hipGetLastError();
-}
-
 hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
                        uint32_t max_bits, hipStream_t st)
 {
@@ -1625,18 +1624,14 @@ hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint
 
 uint32_t cand_words() { return 2 * kCandSlots + 4; }
 
-hipError_t launch_tiles(const TileRun *runs, uint32_t nruns, const CellGroup *cells, TileDesc *tiles, uint32_t ntiles,
-                        hipStream_t st)
+hipError_t launch_prep(const PrepArgs &a, hipStream_t st)
 {
-    if (!ntiles) return hipSuccess;
-    hipLaunchKernelGGL(k_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, st, runs, nruns, cells, tiles, ntiles);
-    return hipGetLastError();
-}
-
-hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st)
-{
-    if (!ncells) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_init, dim3((1u << kMaxBits) / 256u, ncells), dim3(256), 0, st, cells, counts);
+    const uint64_t flat = a.nctrl + a.nheavy + a.ncand + a.nzero + a.ntiles;
+    uint64_t bx = (flat + 255) / 256;
+    if (bx < a.ncells) bx = a.ncells;
+    if (bx > 4096) bx = 4096;
+    if (!bx) return hipSuccess;
+    hipLaunchKernelGGL(k_prep, dim3((uint32_t)bx, a.ncells ? 2u : 1u), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -14,7 +14,6 @@ size_t lds_join();
 uint32_t join_table_cap();
 uint32_t cand_words();   // words of one job's candidate map (keys, counts, found flag)
 
-hipError_t launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
                        uint32_t max_bits, hipStream_t st);
 hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk_ad, uint32_t nblk_ad,
@@ -36,11 +35,10 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 // cand_words() zeroed words at cand_base + cand_off.
 hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
                                uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
-// The tile list from its runs (sorted by first, none empty).
-hipError_t launch_tiles(const TileRun *runs, uint32_t nruns, const CellGroup *cells, TileDesc *tiles, uint32_t ntiles,
-                        hipStream_t st);
-// Cursors of the cell layout: every group's cursor starts at its cell.
-hipError_t launch_cell_init(const CellGroup *cells, uint32_t ncells, uint32_t *counts, hipStream_t st);
+// One launch before a batch's passes: zero the control words, heavy tables,
+// candidate maps and counted relations' counters, expand the tile runs, set
+// the cell cursors.
+hipError_t launch_prep(const PrepArgs &a, hipStream_t st);
 // Work items of every instance, with their partitions' position maps.
 // spart[i] and rpart[i] receive instance i's partitioned S and R tuple counts.
 hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32_t *pstart, const CellGroup *cells,
