@@ -489,7 +489,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // the pre-sample saw a candidate) and one selection launch for all
     if (nheavy_inst)
         SPJ_TRY(launch_heavy_detect(d_insts, d_jobs, (uint32_t)h_jobs_.size(), hblocks, cand_.as<uint32_t>(), nheavy,
-                                    st));
+                                    (uint32_t)std::max(1, device_info().cus), st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[1], st));
 
     // ---- partition: histogram (counted tiles), scan, scatter

@@ -656,66 +656,73 @@ constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
 
 __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
-                                                      uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase)
+                                                      uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase,
+                                                      uint32_t nvb)
 {
     extern __shared__ __align__(16) uint32_t lds[];
     uint32_t *sk = lds;
     uint32_t *sc = lds + kSampleSlots;
-    // this block's job: last job whose first_block[phase] <= blockIdx.x
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        const uint32_t m = (lo + hi) >> 1;
-        if (jobs[m].first_block[phase] <= blockIdx.x) lo = m; else hi = m;
-    }
-    const HeavyJob job = jobs[lo];
-    const InstDesc &I = insts[job.inst];
-    uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
-    uint32_t *found = cand_cnt + kCandSlots;
-    // sample block of this workgroup: phase 0 takes blocks 0, kPreSample,
-    // 2 kPreSample, ...; phase 1 the others
-    const uint32_t l = blockIdx.x - job.first_block[phase];
-    const uint32_t k = phase == 0 ? l * kPreSample : (l / (kPreSample - 1)) * kPreSample + l % (kPreSample - 1) + 1;
-    if (k >= job.nblocks) return;
-    if (phase == 1 && *found == 0) return;   // no candidate in the pre-sample: no skew worth the rest
-    for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
-    __syncthreads();
-    const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
-    const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
-    // all of the block's keys are loaded first (kSampleBlock / 1024 per
-    // thread), so the loads overlap instead of waiting one by one
-    constexpr uint32_t KPT = kSampleBlock / 1024;
-    uint32_t keys[KPT];
-#pragma unroll
-    for (uint32_t u = 0; u < KPT; u++) {
-        const uint64_t i = b + u * 1024 + threadIdx.x;
-        keys[u] = (i < e) ? gld(I.S, i).x : kEmptyKey;
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < KPT; u++) {
-        const uint32_t key = keys[u];
-        if (key == kEmptyKey) continue;
-        uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
-        for (uint32_t step = 0; step < kSampleSlots; step++) {
-            uint32_t old = atomicCAS(&sk[s], kEmptyKey, key);
-            if (old == kEmptyKey || old == key) { atomicAdd(&sc[s], 1u); break; }
-            s = (s + 1u) & (kSampleSlots - 1u);
+    // workgroups loop over the phase's nvb sample blocks (a grid of at most
+    // one workgroup per CU: when phase 1 has nothing to do, its blocks exit
+    // without each being dispatched)
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        __syncthreads();   // the previous sample block's table reads are done
+        // this block's job: last job whose first_block[phase] <= vb
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t m = (lo + hi) >> 1;
+            if (jobs[m].first_block[phase] <= vb) lo = m; else hi = m;
         }
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
-        if (sc[i] < kLocalMin) continue;
-        const uint32_t key = sk[i];
-        uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
-        for (uint32_t step = 0; step < kCandSlots; step++) {
-            uint32_t old = atomicCAS(&cand_keys[s], kEmptyKey, key);
-            if (old == kEmptyKey || old == key) {
-                const uint32_t sum = atomicAdd(&cand_cnt[s], sc[i]) + sc[i];
-                // open the second phase only on a key seen often across the
-                // pre-sample, not on one chance cluster in a single block
-                if (phase == 0 && sum >= kGateMin) *found = 1u;
-                break;
+        const HeavyJob job = jobs[lo];
+        const InstDesc &I = insts[job.inst];
+        uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
+        uint32_t *found = cand_cnt + kCandSlots;
+        // sample block of this workgroup: phase 0 takes blocks 0, kPreSample,
+        // 2 kPreSample, ...; phase 1 the others
+        const uint32_t l = vb - job.first_block[phase];
+        const uint32_t k = phase == 0 ? l * kPreSample : (l / (kPreSample - 1)) * kPreSample + l % (kPreSample - 1) + 1;
+        if (k >= job.nblocks) continue;
+        if (phase == 1 && *found == 0) continue;   // no candidate in the pre-sample: no skew worth the rest
+        for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
+        __syncthreads();
+        const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
+        const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
+        // all of the block's keys are loaded first (kSampleBlock / 1024 per
+        // thread), so the loads overlap instead of waiting one by one
+        constexpr uint32_t KPT = kSampleBlock / 1024;
+        uint32_t keys[KPT];
+#pragma unroll
+        for (uint32_t u = 0; u < KPT; u++) {
+            const uint64_t i = b + u * 1024 + threadIdx.x;
+            keys[u] = (i < e) ? gld(I.S, i).x : kEmptyKey;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < KPT; u++) {
+            const uint32_t key = keys[u];
+            if (key == kEmptyKey) continue;
+            uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
+            for (uint32_t step = 0; step < kSampleSlots; step++) {
+                uint32_t old = atomicCAS(&sk[s], kEmptyKey, key);
+                if (old == kEmptyKey || old == key) { atomicAdd(&sc[s], 1u); break; }
+                s = (s + 1u) & (kSampleSlots - 1u);
             }
-            s = (s + 1u) & (kCandSlots - 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
+            if (sc[i] < kLocalMin) continue;
+            const uint32_t key = sk[i];
+            uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
+            for (uint32_t step = 0; step < kCandSlots; step++) {
+                uint32_t old = atomicCAS(&cand_keys[s], kEmptyKey, key);
+                if (old == kEmptyKey || old == key) {
+                    const uint32_t sum = atomicAdd(&cand_cnt[s], sc[i]) + sc[i];
+                    // open the second phase only on a key seen often across the
+                    // pre-sample, not on one chance cluster in a single block
+                    if (phase == 0 && sum >= kGateMin) *found = 1u;
+                    break;
+                }
+                s = (s + 1u) & (kCandSlots - 1u);
+            }
         }
     }
 }
@@ -731,6 +738,9 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     const InstDesc &I = insts[job.inst];
     const uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
     const uint32_t min_count = job.min_count;
+    // the pre-sample opened no gate and no key's pre-sample count can reach
+    // min_count: no heavy key (the table and count stay zero)
+    if (cand_cnt[kCandSlots] == 0 && min_count >= kGateMin) return;
     if (threadIdx.x == 0) n = 0;
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) tab[i] = kEmptyKey;
     __syncthreads();
@@ -1610,14 +1620,16 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 }
 
 hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
-                               uint32_t *cand_base, uint32_t *nheavy, hipStream_t st)
+                               uint32_t *cand_base, uint32_t *nheavy, uint32_t grid, hipStream_t st)
 {
     if (!njobs || !blocks[0]) return hipSuccess;
     size_t l = 2 * kSampleSlots * 4;
     hipError_t e = set_lds((const void *)k_heavy_sample, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[0]), dim3(1024), l, st, insts, jobs, njobs, cand_base, 0u);
-    if (blocks[1]) hipLaunchKernelGGL(k_heavy_sample, dim3(blocks[1]), dim3(1024), l, st, insts, jobs, njobs, cand_base, 1u);
+    const uint32_t g0 = min(blocks[0], grid), g1 = min(blocks[1], grid);
+    hipLaunchKernelGGL(k_heavy_sample, dim3(g0), dim3(1024), l, st, insts, jobs, njobs, cand_base, 0u, blocks[0]);
+    if (blocks[1])
+        hipLaunchKernelGGL(k_heavy_sample, dim3(g1), dim3(1024), l, st, insts, jobs, njobs, cand_base, 1u, blocks[1]);
     hipLaunchKernelGGL(k_heavy_select, dim3(njobs), dim3(1024), 0, st, insts, jobs, cand_base, nheavy);
     return hipGetLastError();
 }

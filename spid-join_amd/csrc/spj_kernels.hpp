@@ -32,9 +32,10 @@ uint32_t heavy_sample_blocks(uint32_t nS, uint32_t max_blocks, uint32_t *stride)
 // job's pre-sample blocks (blocks[0] in all), a second one over the other
 // sample blocks (blocks[1]; a job skips them when its pre-sample found no
 // candidate), then one selection workgroup per job.  Each job owns
-// cand_words() zeroed words at cand_base + cand_off.
+// cand_words() zeroed words at cand_base + cand_off.  The sampling launches
+// have at most `grid` workgroups (one per CU).
 hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint32_t njobs, const uint32_t blocks[2],
-                               uint32_t *cand_base, uint32_t *nheavy, hipStream_t st);
+                               uint32_t *cand_base, uint32_t *nheavy, uint32_t grid, hipStream_t st);
 // One launch before a batch's passes: zero the control words, heavy tables,
 // candidate maps and counted relations' counters, expand the tile runs, set
 // the cell cursors.
