@@ -734,6 +734,7 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     __shared__ uint32_t n, placed;
     __shared__ uint32_t sel[kHeavyMaxKeys];
     __shared__ __align__(16) uint32_t tab[kHeavySlots];
+    __shared__ uint32_t bfill[kHeavySlots / 4];   // slots taken (or asked for) per bucket
     const HeavyJob job = jobs[blockIdx.x];
     const InstDesc &I = insts[job.inst];
     const uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
@@ -741,8 +742,12 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     // the pre-sample opened no gate and no key's pre-sample count can reach
     // min_count: no heavy key (the table and count stay zero)
     if (cand_cnt[kCandSlots] == 0 && min_count >= kGateMin) return;
-    if (threadIdx.x == 0) n = 0;
+    if (threadIdx.x == 0) {
+        n = 0;
+        placed = 0;
+    }
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) tab[i] = kEmptyKey;
+    for (uint32_t i = threadIdx.x; i < kHeavySlots / 4; i += blockDim.x) bfill[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kCandSlots; i += blockDim.x) {
         if (cand_keys[i] == kEmptyKey || cand_cnt[i] < min_count) continue;
@@ -750,26 +755,24 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
         if (k < kHeavyMaxKeys) sel[k] = cand_keys[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // 2-choice placement into the emptier of the key's two 4-slot buckets;
-        // a key with both buckets full stays an ordinary key
-        uint32_t p = 0;
-        const uint32_t m = min(n, kHeavyMaxKeys);
-        for (uint32_t k = 0; k < m; k++) {
-            const uint32_t key = sel[k];
-            const uint32_t b[2] = {heavy_b1(key), heavy_b2(key)};
-            uint32_t fill[2];
-            for (int c = 0; c < 2; c++) {
-                fill[c] = 0;
-                for (uint32_t s = 0; s < 4; s++) fill[c] += tab[4 * b[c] + s] != kEmptyKey;
-            }
-            const int c = fill[1] < fill[0] ? 1 : 0;
-            if (fill[c] < 4) {
-                tab[4 * b[c] + fill[c]] = key;
-                p++;
-            }
+    // 2-choice placement, one key per thread: a slot of the emptier of the
+    // key's two 4-slot buckets (as seen now), else of the other; a key with
+    // both buckets full stays an ordinary key.  Keys are unique, so each
+    // sits in at most one slot.
+    static_assert(kHeavyMaxKeys <= 1024, "one selected key per thread");
+    if (threadIdx.x < min(n, kHeavyMaxKeys)) {
+        const uint32_t key = sel[threadIdx.x];
+        const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
+        const uint32_t first = bfill[b2] < bfill[b1] ? b2 : b1, second = first == b1 ? b2 : b1;
+        uint32_t slot = atomicAdd(&bfill[first], 1u), b = first;
+        if (slot >= 4 && second != first) {
+            slot = atomicAdd(&bfill[second], 1u);
+            b = second;
         }
-        placed = p;
+        if (slot < 4) {
+            tab[4 * b + slot] = key;
+            atomicAdd(&placed, 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
