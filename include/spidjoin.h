@@ -150,6 +150,29 @@ int spj_finish(spj_instance_t *inst, spj_pair_t **pairs, uint64_t *count);
 int spj_execute(spj_instance_t *inst, const char *join_type, const spj_join_design_t *design, spj_pair_t **pairs,
                 uint64_t *count);
 int spj_last_phase_times(spj_instance_t *inst, spj_phase_times_t *times);
+/* Join plan, replacing IDPHandler::SetQueryTree (pipeline.cc:251-279): a plan
+ * document {"query_tree": [{"operator": ...}, ...]} with the operators of
+ * execution.hpp's join_json / bg_join_json (SEND_DATA_OPT left/right,
+ * SYNC_THREADS, GLB_PARTITION, BG_BROADCAST + CHIPWISE_BROADCAST,
+ * GLB_CHIPWISE_PARTITION, ROTATE_AND_STREAM, RNS_JOIN / IDP_JOIN,
+ * RECV_DATA_OPT).  It is validated (each relation sent, then partitioned
+ * once; one join after both partitions; the result received after it) and
+ * mapped onto the GPU phases: BG_BROADCAST turns heavy-key handling on.
+ * Other operators are rejected (SPJ_EINVAL).  NULL returns to the design's
+ * choice at StartJoin (bank_set or rank_set > 1: the BG_BROADCAST plan). */
+int spj_set_query_plan(spj_instance_t *inst, const char *plan_json);
+/* Validate a plan without an instance: node count and whether it replicates. */
+int spj_plan_check(const char *plan_json, int *num_nodes, int *replicate);
+/* RecordLogs(start, rank_group), queryHostFunc.cc:183-241: the last join's
+ * timeline, one line per plan node
+ *   "rank_group-0-index-start_ms-stop_ms-OPERATOR-inputs"
+ * with times from the CLOCK_MONOTONIC instant `start` (each node spans the
+ * GPU phase doing its work; R's and S's partitions share an interval when
+ * one launch scattered both), then "rnc_xfer_bytes B rnc_num_packets P" for
+ * the partitioned R and S (B bytes moved as P 8-byte tuples).  *len receives
+ * the text length; buf (cap > length) receives the text, or may be NULL. */
+int spj_timeline(spj_instance_t *inst, int64_t start_sec, int64_t start_nsec, int rank_group, char *buf,
+                 uint64_t cap, uint64_t *len);
 void spj_free_result(spj_pair_t *pairs);
 void spj_destroy(spj_instance_t *inst);
 

@@ -502,9 +502,12 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     // one persistent 1024-thread workgroup per CU (LDS-bound); R first
     // when its tiles record heavy rids for the S tiles
     const uint32_t sgrid = (uint32_t)std::max(1, device_info().cus);
-    if (scat_split)
+    if (scat_split) {
         SPJ_TRY(launch_scatter(d_insts, d_tiles, scat_split, counts_.as<uint32_t>(), out_hi, err, ctrl + 4, max_bits_,
                                max_ipt_, sgrid, st));
+        if (timed) SPJ_TRY(hipEventRecord(ev_[7], st));
+    }
+    split_timed_ = timed && scat_split != 0;
     SPJ_TRY(launch_scatter(d_insts, d_tiles + scat_split, ntiles - scat_split, counts_.as<uint32_t>(), out_hi, err,
                            ctrl + 4 + kScatterGroups, max_bits_, max_ipt_, sgrid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
@@ -607,6 +610,8 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         timed->items = ms[4];
         timed->join = ms[5];
         SPJ_TRY(hipEventElapsedTime(&timed->total, ev_[0], ev_[6]));
+        timed->scatter_r = -1;
+        if (split_timed_) SPJ_TRY(hipEventElapsedTime(&timed->scatter_r, ev_[3], ev_[7]));
         if (n) {
             timed->heavy_keys = h_ctrl_[ctrl_at(n, kCtrlHeavy, 0)];
             const InstDesc &d0 = h_insts_[0];

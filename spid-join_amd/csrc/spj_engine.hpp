@@ -45,6 +45,7 @@ struct InstSpec {
 // Milliseconds per phase (HIP events on the context's stream).
 struct PhaseTimes {
     float heavy = 0, hist = 0, scan = 0, scatter = 0, items = 0, join = 0, total = 0;
+    float scatter_r = -1;               // R's own scatter launch (ran first); -1: R and S in one launch
     uint32_t heavy_keys = 0;            // instance 0
     uint64_t r_parted = 0, s_parted = 0;  // instance 0: tuples written to partitions
     uint32_t s_cells = 0, r_cells = 0;    // instance 0: S / R layout was cells
@@ -105,7 +106,8 @@ public:
 
 private:
     hipStream_t stream_ = nullptr;
-    hipEvent_t ev_[8] = {};
+    hipEvent_t ev_[8] = {};   // [0, 6]: phase boundaries; [7]: end of R's own scatter launch
+    bool split_timed_ = false;  // the last timed enqueue recorded ev_[7]
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;

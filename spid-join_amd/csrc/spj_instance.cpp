@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <memory>
 #include <string>
@@ -19,6 +20,7 @@
 #include "spidjoin.h"
 #include "spj_capi_internal.hpp"
 #include "spj_engine.hpp"
+#include "spj_plan.hpp"
 
 struct spj_instance {
     int ranks = 16, rank_groups = 1, ranks_per_group = 16;
@@ -30,9 +32,29 @@ struct spj_instance {
     spj::InstSpec spec;
     spj::PhaseTimes times;
     bool have_times = false;
+    // join plan: set explicitly, else chosen from the design at StartJoin
+    spj::Plan plan, run_plan;
+    bool plan_set = false;
+    // timeline of the last join: CLOCK_MONOTONIC ns of the first H2D copy's
+    // enqueue (t0) and around the result copy; events around the two copies
+    hipEvent_t ev[3] = {};
+    int64_t t0_ns = 0, recv0_ns = 0, recv1_ns = 0;
+    std::vector<spj::NodeTime> node_times;
 };
 
 using namespace spj;
+
+static int64_t now_ns()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (int64_t)ts.tv_sec * 1000000000 + ts.tv_nsec;
+}
+
+// The plan StartJoin runs: the one set, else SetQueryTree's choice
+// (pipeline.cc:251-279): replication (bank_set or rank_set > 1) selects the
+// plan with BG_BROADCAST.
+static bool design_replicates(const spj_join_design_t *d) { return d && (d->bank_set > 1 || d->rank_set > 1); }
 
 static Resolved design_params(const spj_instance *in, const spj_join_design_t *d)
 {
@@ -42,11 +64,51 @@ static Resolved design_params(const spj_instance *in, const spj_join_design_t *d
         if (d->join_alg == 2) c.algorithm = SPJ_ALG_SMJ;
         else if (d->join_alg == 3) c.algorithm = SPJ_ALG_NL;
         else c.algorithm = SPJ_ALG_PHJ;   // 0 phj; 1 nphj is served by the partitioned hash join
-        // SPID's replication switches (bank_set / rank_set > 1, pipeline.cc:251-279)
-        // select the skew-resistant path: heavy-key handling.
-        if (d->bank_set > 1 || d->rank_set > 1) c.heavy = 1;
     }
+    // SPID's replication switches (bank_set / rank_set > 1, pipeline.cc:251-279)
+    // select the skew-resistant plan; its BG_BROADCAST is heavy-key handling
+    if (in->plan_set ? in->plan.replicate : design_replicates(d)) c.heavy = 1;
     return resolve(&c, (uint64_t)in->nR, (uint64_t)in->nS, -1.0);
+}
+
+// Start and stop of every plan node (ms from t0), from the copy events, the
+// GPU phase times (back to back on the instance's stream) and the host
+// clock around the result copy.
+static void plan_times(spj_instance *in, float send_r, float send_s)
+{
+    using spj::PlanOp;
+    const spj::PhaseTimes &t = in->times;
+    const double c = (double)send_r + send_s;        // end of both copies
+    const double h1 = c + t.heavy;                   // end of heavy-key detection
+    const double p0 = in->run_plan.replicate ? h1 : c;   // without BG_BROADCAST the detection counts as partitioning
+    const double pe = h1 + t.hist + t.scan + t.scatter;  // end of partitioning
+    const bool split = t.scatter_r >= 0;
+    const double r_end = split ? h1 + t.hist + t.scan + t.scatter_r : pe;
+    const double s0 = split ? r_end : p0;
+    const double je = pe + t.items + t.join;
+    const double rc0 = in->recv0_ns ? (double)(in->recv0_ns - in->t0_ns) / 1e6 : je;
+    const double rc1 = in->recv1_ns ? (double)(in->recv1_ns - in->t0_ns) / 1e6 : rc0;
+    in->node_times.assign(in->run_plan.nodes.size(), spj::NodeTime());
+    double cur = 0;
+    for (size_t i = 0; i < in->run_plan.nodes.size(); i++) {
+        const spj::PlanNode &n = in->run_plan.nodes[i];
+        double a = cur, b = cur;
+        switch (n.op) {
+        case PlanOp::Send: a = n.side ? send_r : 0.0; b = n.side ? c : send_r; break;
+        case PlanOp::Sync: break;
+        case PlanOp::BgBroadcast: a = c; b = h1; break;
+        case PlanOp::ChipwiseBroadcast: a = b = h1; break;
+        case PlanOp::Partition:
+        case PlanOp::ChipwisePartition: a = n.side ? s0 : p0; b = n.side ? pe : r_end; break;
+        case PlanOp::RotateAndStream: a = b = pe; break;
+        case PlanOp::RnsJoin:
+        case PlanOp::IdpJoin: a = pe; b = je; break;
+        case PlanOp::Recv: a = rc0; b = rc1; break;
+        }
+        in->node_times[i].start_ms = a;
+        in->node_times[i].stop_ms = b;
+        if (b > cur) cur = b;
+    }
 }
 
 extern "C" {
@@ -125,10 +187,19 @@ int spj_start(spj_instance_t *in, const spj_join_design_t *d)
     }
     if (in->started) return fail(SPJ_ESTATE, "StartJoin twice without FinishJoin");
     in->spec.cfg = design_params(in, d);
+    in->run_plan = in->plan_set ? in->plan : spj::default_plan(design_replicates(d));
     hipStream_t st = in->ctx.stream();
     hipError_t e = hipSuccess;
-    if (in->nR) e = hipMemcpyAsync(in->dR.p, in->hR, (size_t)in->nR * 8, hipMemcpyHostToDevice, st);
+    for (int i = 0; i < 3 && e == hipSuccess; i++)
+        if (!in->ev[i]) e = hipEventCreate(&in->ev[i]);
+    if (e != hipSuccess) return hip_fail(e, "StartJoin events");
+    in->t0_ns = now_ns();
+    in->recv0_ns = in->recv1_ns = 0;
+    e = hipEventRecord(in->ev[0], st);
+    if (e == hipSuccess && in->nR) e = hipMemcpyAsync(in->dR.p, in->hR, (size_t)in->nR * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(in->ev[1], st);
     if (e == hipSuccess && in->nS) e = hipMemcpyAsync(in->dS.p, in->hS, (size_t)in->nS * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(in->ev[2], st);
     if (e != hipSuccess) return hip_fail(e, "StartJoin H2D");
     in->spec.R = in->dR.as<uint2>();
     in->spec.S = in->dS.as<uint2>();
@@ -150,6 +221,11 @@ int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
     hipError_t e = in->ctx.collect(1, counts, errs, &in->times);
     if (e != hipSuccess) return hip_fail(e, "FinishJoin");
     in->have_times = true;
+    float send_r = 0, send_s = 0;
+    e = hipEventElapsedTime(&send_r, in->ev[0], in->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&send_s, in->ev[1], in->ev[2]);
+    if (e != hipSuccess) return hip_fail(e, "FinishJoin timing");
+    plan_times(in, send_r, send_s);
     int rc = err_from_flags(errs[0], 0);
     if (rc) return rc;
     uint64_t n = counts[0];
@@ -157,6 +233,7 @@ int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
     size_t bytes = ((n * 8 + 63) / 64) * 64;
     spj_pair_t *buf = (spj_pair_t *)aligned_alloc(64, bytes ? bytes : 64);
     if (!buf) return fail(SPJ_ENOMEM, "result buffer");
+    in->recv0_ns = now_ns();
     if (n) {
         e = hipMemcpy(buf, in->dOut.p, n * 8, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
@@ -164,6 +241,8 @@ int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
             return hip_fail(e, "FinishJoin D2H");
         }
     }
+    in->recv1_ns = now_ns();
+    plan_times(in, send_r, send_s);
     *pairs = buf;
     *count = n;
     return SPJ_OK;
@@ -193,6 +272,47 @@ int spj_last_phase_times(spj_instance_t *in, spj_phase_times_t *t)
     return SPJ_OK;
 }
 
+int spj_set_query_plan(spj_instance_t *in, const char *plan_json)
+{
+    if (!in) return fail(SPJ_EINVAL, "null instance");
+    if (in->started) return fail(SPJ_ESTATE, "plan change while a join is running");
+    if (!plan_json) {   // back to the design's choice
+        in->plan_set = false;
+        return SPJ_OK;
+    }
+    spj::Plan p;
+    int rc = spj::parse_plan(plan_json, &p);
+    if (rc) return rc;
+    in->plan = std::move(p);
+    in->plan_set = true;
+    return SPJ_OK;
+}
+
+int spj_plan_check(const char *plan_json, int *num_nodes, int *replicate)
+{
+    spj::Plan p;
+    int rc = spj::parse_plan(plan_json, &p);
+    if (rc) return rc;
+    if (num_nodes) *num_nodes = (int)p.nodes.size();
+    if (replicate) *replicate = p.replicate ? 1 : 0;
+    return SPJ_OK;
+}
+
+int spj_timeline(spj_instance_t *in, int64_t start_sec, int64_t start_nsec, int rank_group, char *buf, uint64_t cap,
+                 uint64_t *len)
+{
+    if (!in || !len) return fail(SPJ_EINVAL, "null argument");
+    if (!in->have_times || in->node_times.empty()) return fail(SPJ_ESTATE, "no finished join");
+    const std::vector<uint64_t> parted = {in->times.r_parted, in->times.s_parted};
+    const std::string s = spj::timeline_text(in->run_plan, in->node_times, in->t0_ns,
+                                             start_sec * 1000000000 + start_nsec, rank_group, parted);
+    *len = s.size();
+    if (!buf) return SPJ_OK;
+    if (cap <= s.size()) return fail(SPJ_EINVAL, "timeline needs %zu bytes", s.size() + 1);
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return SPJ_OK;
+}
+
 void spj_free_result(spj_pair_t *pairs) { free(pairs); }
 
 void spj_destroy(spj_instance_t *in)
@@ -202,6 +322,8 @@ void spj_destroy(spj_instance_t *in)
     in->dR.release();
     in->dS.release();
     in->dOut.release();
+    for (hipEvent_t e : in->ev)
+        if (e) (void)hipEventDestroy(e);
     delete in;
 }
 

@@ -81,22 +81,18 @@ void JoinInstance::ExecuteJoin(std::string join_type, join_design_t d)
     FinishJoin();
 }
 
-// Timeline of the last join, one line per phase:
-// "rank_group-0-idx-start_ms-stop_ms-PHASE" (queryHostFunc.cc:183-303 prints
-// rg-rank-idx-start-stop-op-inputs per plan node).
+// Timeline of the last join (queryHostFunc.cc:183-241, the stdout branch):
+// one "rank_group-0-idx-start-stop-OPERATOR-inputs" line per plan node, times
+// in ms from `start`, then the partition transfer lines.
 void JoinInstance::RecordLogs(struct timespec start, int rank_group)
 {
-    (void)start;
-    spj_phase_times_t t;
-    if (spj_last_phase_times(h_, &t) != SPJ_OK) return;
-    const char *names[] = {"HEAVY_KEY_DETECT", "GLB_PARTITION_COUNT", "CALCULATE_PAGE_HISTOGRAM",
-                           "GLB_PARTITION_PACKET", "MAKE_WORK_ITEMS", "JOIN"};
-    float v[] = {t.heavy_ms, t.hist_ms, t.scan_ms, t.scatter_ms, t.items_ms, t.join_ms};
-    float at = 0;
-    for (int i = 0; i < 6; i++) {
-        printf("%d-0-%d-%.4f-%.4f-%s\n", rank_group, i, at, at + v[i], names[i]);
-        at += v[i];
-    }
+    uint64_t len = 0;
+    if (spj_timeline(h_, (int64_t)start.tv_sec, (int64_t)start.tv_nsec, rank_group, nullptr, 0, &len) != SPJ_OK) return;
+    std::string text(len + 1, '\0');
+    if (spj_timeline(h_, (int64_t)start.tv_sec, (int64_t)start.tv_nsec, rank_group, &text[0], text.size(), &len) !=
+        SPJ_OK)
+        return;
+    fputs(text.c_str(), stdout);
     fflush(stdout);
 }
 

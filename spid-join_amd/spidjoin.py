@@ -25,7 +25,8 @@ EXPORTS = [
     "spj_config_default", "spj_last_error", "spj_init", "spj_device_info", "spj_dev_alloc", "spj_dev_free",
     "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_pairs_checksum_device", "spj_dev_sync", "spj_join_device", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
-    "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_free_result", "spj_destroy",
+    "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_set_query_plan", "spj_plan_check",
+    "spj_timeline", "spj_free_result", "spj_destroy",
     "spj_tuner_set", "spj_tuner_get", "spj_tuner_clear", "spj_tuner_entries", "spj_tuner_save",
     "spj_tuner_load", "spj_autotune", "spj_gen_r", "spj_gen_s", "spj_server_create", "spj_server_add_table",
     "spj_server_submit", "spj_server_wait", "spj_server_stats", "spj_server_destroy",
@@ -103,6 +104,10 @@ def lib():
         "spj_execute": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.POINTER(JoinDesign), ctypes.POINTER(P),
                                        ctypes.POINTER(U64)]),
         "spj_last_phase_times": (ctypes.c_int, [P, ctypes.POINTER(PhaseTimes)]),
+        "spj_set_query_plan": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "spj_plan_check": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "spj_timeline": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_char_p, U64,
+                                        ctypes.POINTER(U64)]),
         "spj_free_result": (None, [P]),
         "spj_destroy": (None, [P]),
         "spj_tuner_set": (ctypes.c_int, [U64, U64, ctypes.c_double, ctypes.POINTER(Config)]),
@@ -312,6 +317,20 @@ class JoinInstance:
         check(lib().spj_last_phase_times(self.h, ctypes.byref(t)))
         return t.as_dict()
 
+    def set_query_plan(self, plan_json):
+        """The join plan StartJoin runs (None: the design's choice)."""
+        check(lib().spj_set_query_plan(self.h, None if plan_json is None else plan_json.encode()))
+
+    def timeline(self, start_ns=0, rank_group=0):
+        """RecordLogs text of the last join; start_ns is a CLOCK_MONOTONIC instant
+        (time.clock_gettime_ns(time.CLOCK_MONOTONIC))."""
+        n = U64()
+        sec, nsec = divmod(int(start_ns), 10**9)
+        check(lib().spj_timeline(self.h, sec, nsec, rank_group, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check(lib().spj_timeline(self.h, sec, nsec, rank_group, buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
+
     def close(self):
         if self.h:
             lib().spj_destroy(self.h)
@@ -378,3 +397,10 @@ def sort_pairs(p):
     w = (p["r_rid"].astype(np.uint64) << np.uint64(32)) | p["s_rid"].astype(np.uint64)
     w.sort()
     return w
+
+
+def plan_check(plan_json):
+    """Validate a join plan document; returns (number of nodes, replicates)."""
+    n, r = ctypes.c_int(), ctypes.c_int()
+    check(lib().spj_plan_check(plan_json.encode(), ctypes.byref(n), ctypes.byref(r)))
+    return n.value, bool(r.value)

@@ -20,4 +20,6 @@ def test_reference_driver_flow(args):
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     assert "bad 0" in r.stdout
-    assert "JOIN" in r.stdout   # RecordLogs timeline
+    # RecordLogs: one line per node of the plan the design selects
+    assert "-COMPOUND_FUNC_" in r.stdout and "-HOST_FUNC_RECV_DATA_OPT-" in r.stdout
+    assert ("-COMPOUND_FUNC_BG_BROADCAST-" in r.stdout) == ("-B" in args)

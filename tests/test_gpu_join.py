@@ -1,5 +1,8 @@
 """GPU parity: the HIP join through the C ABI against the oracle (bit-exact
 sorted multiset of (R rid, S rid) pairs, exact count)."""
+import re
+import time
+
 import numpy as np
 import pytest
 
@@ -240,6 +243,83 @@ def test_join_instance_lifecycle(gpu, orc):
     assert ji.phase_times()["total_ms"] > 0
     with pytest.raises(sj.SpjError):
         ji.finish()   # FinishJoin without StartJoin
+    ji.close()
+
+
+def parse_timeline(text):
+    """RecordLogs lines: (rank_group, rank, index, start, stop, operator, inputs)
+    per plan node, then (bytes, packets) per partitioned relation."""
+    nodes, rnc = [], []
+    for line in text.splitlines():
+        if line.startswith("rnc_xfer_bytes"):
+            f = line.split()
+            assert f[0] == "rnc_xfer_bytes" and f[2] == "rnc_num_packets", line
+            rnc.append((int(f[1]), int(f[3])))
+            continue
+        rg, rank, idx, a, b, rest = line.split("-", 5)
+        op, inputs = rest.split("-", 1)
+        assert re.fullmatch(r"\d+\.\d{6}", a) and re.fullmatch(r"\d+\.\d{6}", b), line   # %f
+        nodes.append((int(rg), int(rank), int(idx), float(a), float(b), op, inputs))
+    return nodes, rnc
+
+
+def node_inputs(n):
+    if "input" in n:
+        return n["input"]
+    return ",".join(n.get("inputs", []))
+
+
+@pytest.mark.parametrize("replicate", [False, True])
+def test_join_instance_plan_and_timeline(gpu, orc, replicate):
+    """A plan document set on the instance (the two plans of execution.hpp)
+    runs the join (same pairs as the oracle) and RecordLogs prints one line
+    per plan node, in plan order, inside the join's wall time."""
+    import test_plan
+
+    sj = gpu
+    R = sj.gen_r(1 << 18, 45)
+    S = sj.gen_s(1 << 20, R, 1.2, 46)
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    tree = test_plan.bg_tree() if replicate else test_plan.plain_tree()
+    ji.set_query_plan(test_plan.doc(tree))
+    d = sj.JoinInstance.design(join_alg=0)   # bank_set 1: the plan decides
+    print(f"case plan replicate={replicate}", flush=True)
+    t0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
+    ji.start(d)
+    pairs = ji.finish()
+    wall_ms = (time.clock_gettime_ns(time.CLOCK_MONOTONIC) - t0) / 1e6
+    check_equal(sj, orc, R, S, pairs)
+    pt = ji.phase_times()
+    if replicate:
+        assert pt["heavy_keys"] > 0
+    text = ji.timeline(t0, rank_group=3)
+    print(text, flush=True)
+    nodes, rnc = parse_timeline(text)
+    assert [n[5] for n in nodes] == [n["operator"] for n in tree]
+    assert [n[6] for n in nodes] == [node_inputs(n) for n in tree]
+    assert [n[:3] for n in nodes] == [(3, 0, i) for i in range(len(tree))]
+    for n in nodes:
+        assert 0.0 <= n[3] <= n[4] <= wall_ms + 0.01, n
+    by_op = {}
+    for n in nodes:
+        by_op.setdefault(n[5], []).append(n)
+    sends = by_op["HOST_FUNC_SEND_DATA_OPT"]
+    join = [n for n in nodes if n[5].endswith("_JOIN")][0]
+    recv = by_op["HOST_FUNC_RECV_DATA_OPT"][0]
+    assert max(s[4] for s in sends) <= join[3] + 1e-3 <= recv[3] + 2e-3
+    assert join[4] > join[3]
+    assert rnc == [(8 * pt["r_partitioned"], pt["r_partitioned"]), (8 * pt["s_partitioned"], pt["s_partitioned"])]
+    assert pt["r_partitioned"] == len(R) and 0 < pt["s_partitioned"] <= len(S)
+    # without a plan of its own the instance takes the design's: bank_set 8
+    # selects the replicating plan
+    ji.set_query_plan(None)
+    d8 = sj.JoinInstance.design(join_alg=0, bank_set=8)
+    ji.start(d8)
+    check_equal(sj, orc, R, S, ji.finish())
+    ops = [n[5] for n in parse_timeline(ji.timeline(t0))[0]]
+    assert ops == [n["operator"] for n in test_plan.bg_tree()]
     ji.close()
 
 
