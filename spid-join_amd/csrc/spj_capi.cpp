@@ -12,6 +12,7 @@
 
 #include "spidjoin.h"
 #include "spj_capi_internal.hpp"
+#include "spj_stage.hpp"
 #include "spj_engine.hpp"
 #include "spj_kernels.hpp"
 #include "spj_tuner.hpp"
@@ -262,8 +263,11 @@ int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint6
     hipError_t e = hipMalloc(&dR, nR * 8 + 8);
     if (e == hipSuccess) e = hipMalloc(&dS, nS * 8 + 8);
     if (e == hipSuccess) e = hipMalloc(&dO, nS * 8 + 8);
-    if (e == hipSuccess && nR) e = hipMemcpy(dR, R, nR * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && nS) e = hipMemcpy(dS, S, nS * 8, hipMemcpyHostToDevice);
+    // page-locked staging ring on the null stream, then a wait: the join
+    // runs on another stream
+    if (e == hipSuccess) e = stage_h2d(dR, R, nR * 8, nullptr);
+    if (e == hipSuccess) e = stage_h2d(dS, S, nS * 8, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         rc = hip_fail(e, "spj_join_host staging");
     } else {
@@ -274,7 +278,7 @@ int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint6
             if (cnt > out_cap) {
                 rc = fail(SPJ_EOVERFLOW, "out_cap %llu < %llu pairs", (unsigned long long)out_cap, (unsigned long long)cnt);
             } else {
-                e = cnt ? hipMemcpy(out, dO, cnt * 8, hipMemcpyDeviceToHost) : hipSuccess;
+                e = stage_d2h(out, dO, cnt * 8, nullptr);
                 if (e != hipSuccess) rc = hip_fail(e, "D2H");
                 *out_count = cnt;
             }

@@ -21,6 +21,7 @@
 #include "spj_capi_internal.hpp"
 #include "spj_engine.hpp"
 #include "spj_plan.hpp"
+#include "spj_stage.hpp"
 
 struct spj_instance {
     int ranks = 16, rank_groups = 1, ranks_per_group = 16;
@@ -195,10 +196,12 @@ int spj_start(spj_instance_t *in, const spj_join_design_t *d)
     if (e != hipSuccess) return hip_fail(e, "StartJoin events");
     in->t0_ns = now_ns();
     in->recv0_ns = in->recv1_ns = 0;
+    // through the page-locked staging ring: returns once the columns have
+    // been read (the DMA of the last chunk may still run)
     e = hipEventRecord(in->ev[0], st);
-    if (e == hipSuccess && in->nR) e = hipMemcpyAsync(in->dR.p, in->hR, (size_t)in->nR * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = stage_h2d(in->dR.p, in->hR, (size_t)in->nR * 8, st);
     if (e == hipSuccess) e = hipEventRecord(in->ev[1], st);
-    if (e == hipSuccess && in->nS) e = hipMemcpyAsync(in->dS.p, in->hS, (size_t)in->nS * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = stage_h2d(in->dS.p, in->hS, (size_t)in->nS * 8, st);
     if (e == hipSuccess) e = hipEventRecord(in->ev[2], st);
     if (e != hipSuccess) return hip_fail(e, "StartJoin H2D");
     in->spec.R = in->dR.as<uint2>();
@@ -230,12 +233,11 @@ int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
     if (rc) return rc;
     uint64_t n = counts[0];
     // aligned_alloc like the reference's result buffers (queryHostFunc.cc:1089)
-    size_t bytes = ((n * 8 + 63) / 64) * 64;
-    spj_pair_t *buf = (spj_pair_t *)aligned_alloc(64, bytes ? bytes : 64);
+    spj_pair_t *buf = (spj_pair_t *)alloc_result_buffer(n * 8);
     if (!buf) return fail(SPJ_ENOMEM, "result buffer");
     in->recv0_ns = now_ns();
     if (n) {
-        e = hipMemcpy(buf, in->dOut.p, n * 8, hipMemcpyDeviceToHost);
+        e = stage_d2h(buf, in->dOut.p, n * 8, in->ctx.stream());
         if (e != hipSuccess) {
             free(buf);
             return hip_fail(e, "FinishJoin D2H");

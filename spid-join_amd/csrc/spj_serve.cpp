@@ -29,6 +29,7 @@
 
 #include "spidjoin.h"
 #include "spj_capi_internal.hpp"
+#include "spj_stage.hpp"
 #include "spj_engine.hpp"
 
 namespace {
@@ -101,7 +102,7 @@ void run_batch(spj_server *sv, Slot *sl)
     for (Request *r : b) {
         if (e != hipSuccess) break;
         const Table &t = *sv->tables[(size_t)r->table];
-        if (r->nS) e = hipMemcpyAsync(sl->S.as<uint2>() + off, r->S, r->nS * 8, hipMemcpyHostToDevice, st);
+        e = spj::stage_h2d(sl->S.as<uint2>() + off, r->S, r->nS * 8, st);
         spj::InstSpec s;
         s.R = nullptr;
         s.nR = t.nR;
@@ -125,12 +126,11 @@ void run_batch(spj_server *sv, Slot *sl)
         if (rc == SPJ_OK) r->rc = spj::err_from_flags(errs[i], i);
         if (r->rc == SPJ_OK) {
             r->count = counts[i];
-            size_t bytes = ((r->count * 8 + 63) / 64) * 64;
-            r->pairs = (spj_pair_t *)aligned_alloc(64, bytes ? bytes : 64);
+            r->pairs = (spj_pair_t *)spj::alloc_result_buffer(r->count * 8);
             if (!r->pairs) {
                 r->rc = SPJ_ENOMEM;
             } else if (r->count) {
-                hipError_t e2 = hipMemcpy(r->pairs, sl->out.as<uint2>() + off, r->count * 8, hipMemcpyDeviceToHost);
+                hipError_t e2 = spj::stage_d2h(r->pairs, sl->out.as<uint2>() + off, r->count * 8, st);
                 if (e2 != hipSuccess) r->rc = spj::hip_fail(e2, "serving D2H");
             }
         }
@@ -242,7 +242,8 @@ int spj_server_add_table(spj_server_t *sv, const spj_tuple_t *R, uint64_t nR, in
     spj::DevBuf dR, dummy;
     hipError_t e = dR.ensure(nR * 8 + 8);
     if (e == hipSuccess) e = dummy.ensure(64);
-    if (e == hipSuccess && nR) e = hipMemcpy(dR.p, R, nR * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = spj::stage_h2d(dR.p, R, nR * 8, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     spj::InstSpec s;
     s.R = dR.as<uint2>();
     s.nR = (uint32_t)nR;

@@ -247,6 +247,16 @@ def join(R, S, cfg=None, timed=False):
         dO.free()
 
 
+def join_host(R, S, cfg=None):
+    """spj_join_host: host relations in, exact pairs out (staged copies)."""
+    out = np.empty(max(len(S), 1), dtype=PAIR)
+    n = U64()
+    c = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().spj_join_host(R.ctypes.data if len(R) else None, len(R), S.ctypes.data if len(S) else None, len(S),
+                              out.ctypes.data, len(S), ctypes.byref(n), c))
+    return out[:n.value]
+
+
 def join_batch(Rs, Ss, cfg=None, timed=False):
     """Join many instances in one launch sequence; returns ([pairs_i], times)."""
     n = len(Rs)

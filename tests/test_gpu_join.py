@@ -246,6 +246,25 @@ def test_join_instance_lifecycle(gpu, orc):
     ji.close()
 
 
+@pytest.mark.parametrize("nR,nS,zipf", [(5000, 12345, 0.3), ((1 << 22) + 3, (1 << 24) + 5, 1.0), (777, 0, 0.0)])
+def test_host_paths_staged_copies(gpu, orc, nR, nS, zipf):
+    """Host relations through the page-locked staging ring (chunks of 32 MiB,
+    four slots): below one chunk, several ring turns with a ragged last
+    chunk, and an empty S; both the one-call host join and JoinInstance."""
+    sj = gpu
+    R = sj.gen_r(nR, 71 + nR)
+    S = sj.gen_s(nS, R, zipf, 72 + nS) if nS else np.zeros(0, dtype=R.dtype)
+    print(f"case host nR={nR} nS={nS}", flush=True)
+    check_equal(sj, orc, R, S, sj.join_host(R, S))
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    d = sj.JoinInstance.design(join_alg=0)
+    ji.start(d)
+    check_equal(sj, orc, R, S, ji.finish())
+    ji.close()
+
+
 def parse_timeline(text):
     """RecordLogs lines: (rank_group, rank, index, start, stop, operator, inputs)
     per plan node, then (bytes, packets) per partitioned relation."""
