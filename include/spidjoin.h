@@ -47,7 +47,9 @@ enum { SPJ_ALG_AUTO = -1, SPJ_ALG_PHJ = 0, SPJ_ALG_NPHJ = 1, SPJ_ALG_SMJ = 2, SP
 
 /* Tunables; -1 = automatic (tuner table, else built-in defaults). */
 typedef struct {
-    int32_t algorithm;        /* SPJ_ALG_*: AUTO = hash, nested loop for tiny partitions */
+    int32_t algorithm;        /* SPJ_ALG_*: AUTO = hash, nested loop for tiny partitions;
+                                 NPHJ = no partition pass: one open-addressing table of
+                                 >= 2|R| 8-byte slots in device memory, probed by every S tuple */
     int32_t radix_bits;       /* partition-pass radix bits (0..12) */
     int32_t table_log2;       /* LDS hash-table buckets, log2 (6..13; 12 = the reference's 4096) */
     int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
@@ -130,7 +132,7 @@ typedef struct {
     int32_t rs_ratio;
     int32_t rank_set;   /* > 1: rank-set replication -> heavy-key handling on */
     int32_t bank_set;   /* > 1: bank-set replication -> heavy-key handling on */
-    int32_t join_alg;   /* 0 phj, 1 nphj (served by phj), 2 smj, 3 nl */
+    int32_t join_alg;   /* 0 phj, 1 nphj (one HBM table, no partition pass), 2 smj, 3 nl */
 } spj_join_design_t;
 
 void spj_join_design_default(spj_join_design_t *d);   /* idpHandler.hpp:133-144 defaults */

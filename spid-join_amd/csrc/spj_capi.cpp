@@ -50,7 +50,7 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
 {
     Resolved r = tuned_params((uint32_t)nR, (uint32_t)nS, skew_hint);
     if (!cfg) return r;
-    if (cfg->algorithm >= 0) r.alg = (cfg->algorithm == SPJ_ALG_NPHJ) ? 0u : (uint32_t)cfg->algorithm;
+    if (cfg->algorithm >= 0) r.alg = (uint32_t)cfg->algorithm;
     if (cfg->radix_bits >= 0) r.bits = (uint32_t)cfg->radix_bits;
     if (cfg->table_log2 >= 0) r.table_log2 = (uint32_t)cfg->table_log2;
     if (cfg->heavy >= 0) r.heavy = (uint32_t)cfg->heavy;
@@ -67,7 +67,7 @@ int check_resolved(const Resolved &r)
 {
     if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
     if (r.table_log2 < 6 || r.table_log2 > 13) return fail(SPJ_EINVAL, "table_log2 %u outside 6..13", r.table_log2);
-    if (!(r.alg == 0 || r.alg == 2 || r.alg == 3)) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
+    if (r.alg > 3) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
     if (!(r.ipt == 4 || r.ipt == 8 || r.ipt == 10 || r.ipt == 12))
         return fail(SPJ_EINVAL, "scatter_ipt %u (4, 8, 10 or 12)", r.ipt);
     if (r.tile < 512) return fail(SPJ_EINVAL, "tile_tuples %u < 512", r.tile);

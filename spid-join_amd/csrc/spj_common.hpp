@@ -173,6 +173,21 @@ struct HeavyJob {
 };
 constexpr uint32_t kPreSample = 16;
 
+// One no-partition hash join (join_alg 1): instance `inst`'s R in an open-
+// addressing table of 2^log2cap 8-byte slots at tables + table.  Its blocks
+// in the build and probe launches start at first_build and first_probe
+// (kNphjBuildTuples R and kNphjProbeTuples S tuples per block).
+struct NphjJob {
+    uint32_t inst;
+    uint32_t log2cap;
+    uint64_t table;
+    uint32_t first_build;
+    uint32_t first_probe;
+};
+constexpr uint32_t kNphjPPT = 16;                       // S tuples per thread (256-thread blocks)
+constexpr uint32_t kNphjProbeTuples = 256 * kNphjPPT;
+constexpr uint32_t kNphjBuildTuples = 256 * 4;
+
 // One join work item: partition `part` of instance `inst`, S range
 // [sb, se) and the partition's R range [rb, re), absolute positions in the
 // instance's partitioned regions (so the join reads one descriptor, not the

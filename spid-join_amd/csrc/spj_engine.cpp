@@ -217,18 +217,21 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         max_table_ = std::max(max_table_, c.table_log2);
         max_ipt_ = std::max<int>(max_ipt_, (int)c.ipt);
         const bool pre = s.Rp_pre != nullptr;
+        // no-partition hash join: no heavy keys, no partition tiles (its
+        // partitions stay empty, so it has no work items either)
+        const bool nphj = c.alg == 1 && !pre;
         // cells: no histogram pass.  Cursors are 32-bit, so a relation's
         // cell region plus all its tuples stays below 2^32.  Automatically
         // for large hash-join relations; for S only with heavy-key handling,
         // which takes the keys frequent enough to overflow a cell (R keys
         // are unique).
-        const bool heavy_on = c.heavy && s.nS > 0 && !pre;
+        const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj;
         auto use_cells = [&](uint32_t len, bool skew_safe) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
                    (c.cells == 1 || (c.cells == 2 && c.alg == 0 && len >= (1u << 20) && skew_safe));
         };
-        cells[0][i] = !pre && !s.r_contiguous && use_cells(s.nR, true);
-        cells[1][i] = use_cells(s.nS, heavy_on);
+        cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true);
+        cells[1][i] = !nphj && use_cells(s.nS, heavy_on);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
@@ -238,7 +241,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         d.bits = c.bits;
         d.heavy = heavy_on ? 1u : 0u;
         d.table_log2 = c.table_log2;
-        d.alg = c.alg;
+        d.alg = (c.alg == 1 && !nphj) ? 0u : c.alg;   // a resident table is partitioned: hash join
         d.pstartR = pst_total;
         d.pstartS = pst_total + F + 1;
         pst_total += 2ull * (F + 1);
@@ -249,7 +252,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         }
         items_bound += F + (uint64_t)s.nS / smax + 1;
         for (uint32_t rel = pre ? 1u : 0u; rel < 2; rel++) {
-            const uint32_t len = rel ? s.nS : s.nR;
+            const uint32_t len = nphj ? 0u : (rel ? s.nS : s.nR);
             const bool cl = cells[rel][i];
             // cells: two scatter rounds per tile, handed out by the scatter's
             // tile queues in small steps so the workgroups finish together;
@@ -345,7 +348,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
             uint64_t &total = rel ? sp_total : rp_total;
             (rel ? sp_off : rp_off)[i] = total;
             if (!cells[rel][i]) {
-                if (rel == 1 || !specs[i].Rp_pre) total += rel ? d.nS : d.nR;
+                if (d.alg != 1 && (rel == 1 || !specs[i].Rp_pre)) total += rel ? d.nS : d.nR;
                 continue;
             }
             uint64_t region = 0;
@@ -388,6 +391,25 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         h_jobs_.push_back(j);
     }
 
+    // ---- no-partition hash joins: one table of >= 2|R| slots each
+    h_nphj_.clear();
+    uint64_t nphj_words = 0, nphj_bb = 0, nphj_pb = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (h_insts_[i].alg != 1 || specs[i].Rp_pre) continue;
+        NphjJob j{};
+        j.inst = (uint32_t)i;
+        j.log2cap = 10;
+        while ((1ull << j.log2cap) < 2ull * specs[i].nR) j.log2cap++;
+        j.table = nphj_words;
+        j.first_build = (uint32_t)nphj_bb;
+        j.first_probe = (uint32_t)nphj_pb;
+        nphj_words += 1ull << j.log2cap;
+        nphj_bb += (specs[i].nR + kNphjBuildTuples - 1) / kNphjBuildTuples;
+        nphj_pb += (specs[i].nS + kNphjProbeTuples - 1) / kNphjProbeTuples;
+        h_nphj_.push_back(j);
+    }
+    if (nphj_bb > 0x7FFFFFFFull || nphj_pb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+
     // ---- buffers
     // descriptor block layout (256-byte aligned arrays)
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -398,7 +420,8 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     const size_t o_b = o_ad + al(h_ad_.size() * sizeof(ScanBlock));
     const size_t o_jobs = o_b + al(h_b_.size() * sizeof(ScanBlock));
     const size_t o_cells = o_jobs + al(h_jobs_.size() * sizeof(HeavyJob));
-    const size_t desc_bytes = o_cells + al(h_cells_.size() * sizeof(CellGroup));
+    const size_t o_nphj = o_cells + al(h_cells_.size() * sizeof(CellGroup));
+    const size_t desc_bytes = o_nphj + al(h_nphj_.size() * sizeof(NphjJob));
     SPJ_TRY(desc_.ensure(desc_bytes));
     SPJ_TRY(counts_.ensure(cnt_total * 4));
     SPJ_TRY(part_.ensure(part_total * 4));
@@ -410,6 +433,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     SPJ_TRY(holes_.ensure(items_bound * sizeof(uint4)));
     max_holes_ = (uint32_t)std::min<uint64_t>(items_bound, 0xFFFFFFFFull);
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
+    if (nphj_words) SPJ_TRY(nphj_.ensure(nphj_words * 8));
     if (nheavy_inst) {
         SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
         SPJ_TRY(cand_.ensure((size_t)nheavy_inst * cand_words() * 4));
@@ -439,6 +463,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
         if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
         if (!h_cells_.empty()) memcpy(h + o_cells, h_cells_.data(), h_cells_.size() * sizeof(CellGroup));
+        if (!h_nphj_.empty()) memcpy(h + o_nphj, h_nphj_.data(), h_nphj_.size() * sizeof(NphjJob));
     }
     SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
     SPJ_TRY(hipEventRecord(desc_ev_, st));
@@ -520,6 +545,9 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
     SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
                         holes_.as<uint4>(), max_holes_, grid, st));
+    SPJ_TRY(launch_nphj(d_insts, reinterpret_cast<const NphjJob *>(dd + o_nphj), (uint32_t)h_nphj_.size(),
+                        nphj_.as<uint64_t>(), nphj_words, (uint32_t)nphj_bb, (uint32_t)nphj_pb, out_lo, out_fill,
+                        err, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
     specs_n_ = n;
     h_ns_.resize(n);

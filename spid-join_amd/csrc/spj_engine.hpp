@@ -111,11 +111,12 @@ private:
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_, nphj_;
     PinBuf pin_desc_, pin_ctrl_;
     hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
     bool desc_pending_ = false;
     std::vector<HeavyJob> h_jobs_;
+    std::vector<NphjJob> h_nphj_;   // no-partition hash joins of the batch
     uint32_t max_holes_ = 0;
     size_t specs_n_ = 0;
     std::vector<uint32_t> h_ns_;

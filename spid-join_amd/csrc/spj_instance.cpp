@@ -64,7 +64,8 @@ static Resolved design_params(const spj_instance *in, const spj_join_design_t *d
     if (d) {
         if (d->join_alg == 2) c.algorithm = SPJ_ALG_SMJ;
         else if (d->join_alg == 3) c.algorithm = SPJ_ALG_NL;
-        else c.algorithm = SPJ_ALG_PHJ;   // 0 phj; 1 nphj is served by the partitioned hash join
+        else if (d->join_alg == 1) c.algorithm = SPJ_ALG_NPHJ;
+        else c.algorithm = SPJ_ALG_PHJ;
     }
     // SPID's replication switches (bank_set / rank_set > 1, pipeline.cc:251-279)
     // select the skew-resistant plan; its BG_BROADCAST is heavy-key handling

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "spj_common.hpp"
 #include "spj_kernels.hpp"
 
@@ -1479,6 +1481,105 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 }
 
 // Copy segments of pairs: seg = (src offset, dst offset, length) in pairs.
+// ---------------------------------------------------------------------------
+// No-partition hash join (packetwise_no_partitioned_hash_join_build.c /
+// _probe.c: one table over the whole R, probed by every S tuple).  Here the
+// table is one open-addressing array in HBM per instance, 2^log2cap >= 2|R|
+// slots of (rid << 32 | key), linear probing from a multiplicative hash;
+// key 0 marks an empty slot (key 0 never joins).  Small tables stay in L2
+// and the Infinity Cache, which is where this path pays: no partition pass.
+// Grid (blocks over tuples, jobs).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t nphj_slot(uint32_t key, uint32_t log2cap)
+{
+    return (key * 0x9E3779B1u) >> (32 - log2cap);
+}
+
+// the job owning block b of a launch: the last job whose first block <= b
+__device__ __forceinline__ uint32_t nphj_job_of(const NphjJob *jobs, uint32_t njobs, uint32_t b, bool probe)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if ((probe ? jobs[m].first_probe : jobs[m].first_build) <= b) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_nphj_build(const InstDesc *__restrict__ insts, const NphjJob *__restrict__ jobs,
+                                                    uint32_t njobs, unsigned long long *__restrict__ tables,
+                                                    uint32_t *__restrict__ err)
+{
+    const NphjJob job = jobs[nphj_job_of(jobs, njobs, blockIdx.x, false)];
+    const InstDesc &I = insts[job.inst];
+    unsigned long long *tab = tables + job.table;
+    const uint32_t mask = (1u << job.log2cap) - 1u;
+    const uint64_t b0 = (uint64_t)(blockIdx.x - job.first_build) * kNphjBuildTuples;
+    for (uint32_t u = 0; u < kNphjBuildTuples / 256; u++) {
+        const uint64_t i = b0 + u * 256 + threadIdx.x;
+        if (i >= I.nR) break;
+        const uint2 t = gld(I.R, i);
+        if (t.x == kEmptyKey) continue;
+        const unsigned long long v = ((unsigned long long)t.y << 32) | t.x;
+        // at most |R| of the >= 2|R| slots are taken: an empty slot is found
+        for (uint32_t h = nphj_slot(t.x, job.log2cap);; h = (h + 1u) & mask) {
+            const unsigned long long old = atomicCAS(&tab[h], 0ull, v);
+            if (old == 0ull) break;
+            if ((uint32_t)old == t.x) {   // a2a_srj_build.c:265-271: duplicate R key
+                atomicOr(&err[job.inst], kErrDuplicateKey);
+                break;
+            }
+        }
+    }
+}
+
+// Each block probes kNphjProbeTuples S tuples and reserves their output
+// with one atomic (a block scan of the threads' hit counts gives each thread
+// its place): per-wave reservations would put one atomic per 64 tuples on
+// the same counter.
+__global__ __launch_bounds__(256) void k_nphj_probe(const InstDesc *__restrict__ insts, const NphjJob *__restrict__ jobs,
+                                                    uint32_t njobs, const unsigned long long *__restrict__ tables,
+                                                    uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill)
+{
+    __shared__ uint32_t cnt[256], wtmp[256 / kWave + 1], base;
+    const NphjJob job = jobs[nphj_job_of(jobs, njobs, blockIdx.x, true)];
+    const InstDesc &I = insts[job.inst];
+    const unsigned long long *tab = tables + job.table;
+    const uint32_t mask = (1u << job.log2cap) - 1u;
+    const uint64_t b0 = (uint64_t)(blockIdx.x - job.first_probe) * kNphjProbeTuples;
+    uint2 t[kNphjPPT];
+#pragma unroll
+    for (uint32_t u = 0; u < kNphjPPT; u++) {
+        const uint64_t i = b0 + u * 256 + threadIdx.x;
+        t[u] = i < I.nS ? gld(I.S, i) : make_uint2(kEmptyKey, 0u);
+    }
+    uint32_t hits = 0, n = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kNphjPPT; u++) {
+        if (t[u].x == kEmptyKey) continue;
+        for (uint32_t h = nphj_slot(t[u].x, job.log2cap);; h = (h + 1u) & mask) {
+            const unsigned long long v = tab[h];
+            if (v == 0ull) break;
+            if ((uint32_t)v == t[u].x) {
+                hits |= 1u << u;
+                t[u].x = (uint32_t)(v >> 32);   // the R rid replaces the key
+                n++;
+                break;
+            }
+        }
+    }
+    cnt[threadIdx.x] = n;
+    const uint32_t total = block_exclusive_scan(cnt, 256, wtmp);   // cnt[t] = thread t's first place
+    if (total == 0) return;   // the same in every thread
+    if (threadIdx.x == 0) base = atomicAdd(&out_lo[job.inst], total);
+    __syncthreads();
+    uint32_t o = base + cnt[threadIdx.x];
+#pragma unroll
+    for (uint32_t u = 0; u < kNphjPPT; u++)
+        if (hits & (1u << u)) gst(I.out, o++, t[u]);   // (R rid, S rid)
+    if (threadIdx.x == 0) atomicAdd(&out_fill[job.inst], total);
+}
+
 __global__ void k_gather_segments(const uint2 *__restrict__ src, uint2 *__restrict__ dst, const uint4 *__restrict__ segs,
                                   uint32_t nsegs)
 {
@@ -1669,6 +1770,22 @@ hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *c
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
                        holes, max_holes);
+    return hipGetLastError();
+}
+
+hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njobs, uint64_t *tables,
+                       uint64_t table_words, uint32_t build_blocks, uint32_t probe_blocks, uint32_t *out_lo,
+                       uint32_t *out_fill, uint32_t *err, hipStream_t st)
+{
+    if (!njobs) return hipSuccess;
+    hipError_t e = hipMemsetAsync(tables, 0, table_words * 8, st);
+    if (e != hipSuccess) return e;
+    auto *t = reinterpret_cast<unsigned long long *>(tables);
+    if (build_blocks)
+        hipLaunchKernelGGL(k_nphj_build, dim3(build_blocks), dim3(256), 0, st, insts, jobs, njobs, t, err);
+    if (probe_blocks)
+        hipLaunchKernelGGL(k_nphj_probe, dim3(probe_blocks), dim3(256), 0, st, insts, jobs, njobs,
+                           (const unsigned long long *)t, out_lo, out_fill);
     return hipGetLastError();
 }
 

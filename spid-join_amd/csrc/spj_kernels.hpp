@@ -48,6 +48,12 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
 hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
                        uint32_t grid, hipStream_t st);
+// No-partition hash join of every job's instance: zero the tables, insert
+// R (a duplicate key flags kErrDuplicateKey), then probe S and append each
+// match at out_lo[inst] (out_fill advanced by the same count).
+hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njobs, uint64_t *tables,
+                       uint64_t table_words, uint32_t build_blocks, uint32_t probe_blocks, uint32_t *out_lo,
+                       uint32_t *out_fill, uint32_t *err, hipStream_t st);
 hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st);
 hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *segs, uint32_t nsegs, hipStream_t st);
 

@@ -126,6 +126,68 @@ def test_duplicate_r_key_is_an_error(gpu):
     assert e.value.code == -4
 
 
+@pytest.mark.parametrize("nR,ratio,zipf", [(1 << 12, 2, 0.0), (1 << 16, 4, 1.0), (1 << 20, 2, 0.5),
+                                           (100_003, 3, 1.5), (1, 5, 0.0), (1 << 21, 8, 0.0)])
+def test_no_partition_hash_join(gpu, orc, nR, ratio, zipf):
+    """join_alg 1 (NPHJ): one HBM table over R, probed by every S tuple."""
+    sj = gpu
+    R = sj.gen_r(nR, 300 + nR)
+    S = sj.gen_s(nR * ratio, R, zipf, 400 + nR)
+    print(f"case nphj nR={nR} nS={len(S)} zipf={zipf}", flush=True)
+    pairs, t = sj.join(R, S, sj.config(algorithm=1), timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    assert t["heavy_keys"] == 0 and t["s_partitioned"] == 0 and t["r_partitioned"] == 0
+
+
+def test_no_partition_hash_join_edges(gpu, orc):
+    """Empty R or S, S keys absent from R (unmatched S tuples leave no
+    holes), key 0 in both, and a duplicate R key (an error)."""
+    sj = gpu
+    cfg = sj.config(algorithm=1)
+    R = sj.gen_r(5000, 81)
+    S = sj.gen_s(20000, R, 0.7, 82)
+    empty = np.zeros(0, dtype=R.dtype)
+    for r, s_ in [(empty, S), (R, empty)]:
+        pairs, _ = sj.join(r, s_, cfg)
+        assert len(pairs) == 0
+    other = sj.gen_r(7000, 83)   # keys 1..7000: those above 5000 are not in R
+    S2 = S.copy()
+    S2["key"][::3] = other["key"][: len(S2[::3])]
+    S2["key"][5] = 0
+    R2 = R.copy()
+    R2["key"][7] = 0
+    pairs, _ = sj.join(R2, S2, cfg)
+    check_equal(sj, orc, R2, S2, pairs)
+    R3 = R.copy()
+    R3["key"][10] = R3["key"][11]
+    with pytest.raises(sj.SpjError) as e:
+        sj.join(R3, S, cfg)
+    assert e.value.code == -4
+
+
+def test_no_partition_hash_join_batch_and_instance(gpu, orc):
+    """NPHJ instances of mixed sizes in one batch, and JoinInstance with
+    join_alg 1."""
+    sj = gpu
+    rng = np.random.default_rng(19)
+    Rs, Ss = [], []
+    for i in range(24):
+        nR = int(2 ** rng.uniform(4, 17))
+        R = sj.gen_r(nR, 500 + i)
+        Rs.append(R)
+        Ss.append(sj.gen_s(nR * int(rng.choice([1, 2, 4, 8])), R, float(rng.uniform(0, 1.25)), 600 + i))
+    outs, _ = sj.join_batch(Rs, Ss, sj.config(algorithm=1))
+    for R, S, p in zip(Rs, Ss, outs):
+        check_equal(sj, orc, R, S, p)
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(Rs[3], "left")
+    ji.load_column(Ss[3], "right")
+    d = sj.JoinInstance.design(join_alg=1)
+    ji.start(d)
+    check_equal(sj, orc, Rs[3], Ss[3], ji.finish())
+    ji.close()
+
+
 @pytest.mark.parametrize("bits", [0, 3, 9, 12])
 def test_radix_bits_sweep(gpu, orc, bits):
     """Any radix fan-out gives the same multiset (large partitions are split
