@@ -347,7 +347,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the GPU's clocks take ~8 joins of sustained load to settle (per-call
+    # times fall ~7% over them), so the default warm-up covers that
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--extra", action="store_true", help="also run the skewed 2^26 x 2^28 and sort-merge runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -390,12 +392,12 @@ def main():
              "join_roofline_frac": round(b_alg / (ms * 1e-3) / HBM_PEAK, 4),
              "phases_ms": {k: round(v, 4) for k, v in res["phases"].items()}}
     if args.extra:
-        smj = run_workload(nR, nS, zipf, max(3, args.steps // 4), 1, sj.config(algorithm=sj.ALG_SMJ), 1000, keep)
+        smj = run_workload(nR, nS, zipf, max(3, args.steps // 4), 2, sj.config(algorithm=sj.ALG_SMJ), 1000, keep)
         extra["smj"] = {"ms_per_step": round(smj["ms_per_step"], 4),
                         "tuples_per_s": round((nR + nS) / (smj["ms_per_step"] * 1e-3), 1)}
         sk = {}
         for z in (0.0, 1.0, 1.5):
-            r = run_workload(1 << 26, 1 << 28, z, max(3, args.steps // 4), 1, None, 2000, keep)
+            r = run_workload(1 << 26, 1 << 28, z, max(10, args.steps // 2), args.warmup, None, 2000, keep)
             bal = 24 * ((1 << 26) + (1 << 28)) + 8 * r["pairs"]
             sk[str(z)] = {"ms_per_step": round(r["ms_per_step"], 4),
                           "tuples_per_s": round(((1 << 26) + (1 << 28)) / (r["ms_per_step"] * 1e-3), 1),
