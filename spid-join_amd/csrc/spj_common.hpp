@@ -28,8 +28,8 @@ namespace spj {
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kEmptyKey = 0u;          // key 0 = empty / skipped (glb_partition_count.c:192)
 constexpr uint32_t kNoRid = 0xFFFFFFFFu;    // heavy key with no R tuple yet
-constexpr uint32_t kHeavySlots = 2048;      // heavy (key -> R rid) table, LDS resident
-constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
+constexpr uint32_t kHeavySlots = 4096;      // heavy (key -> R rid) table, LDS resident
+constexpr uint32_t kHeavyMaxKeys = 1024;    // at most a quarter full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
 constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
 constexpr uint32_t kScatterGroups = 8;      // scatter groups (one per XCD) per relation

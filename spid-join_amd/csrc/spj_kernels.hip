@@ -86,7 +86,8 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_bucket
 // Heavy table: kHeavySlots / 4 buckets of 4 slots; a key lives in one of
 // two buckets (2-choice), so a lookup is two independent 16-byte LDS reads
 // and never loops.
-constexpr uint32_t kHeavyBucketBits = 9;   // 512 buckets x 4 slots = kHeavySlots
+constexpr uint32_t kHeavyBucketBits = 10;   // 1024 buckets x 4 slots = kHeavySlots
+static_assert(kHeavySlots == (4u << kHeavyBucketBits), "heavy table: 4-slot buckets");
 __device__ __forceinline__ uint32_t heavy_b1(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - kHeavyBucketBits); }
 __device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
 {
@@ -729,12 +730,31 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
     }
 }
 
+__device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* power of two */)
+{
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t x = a[i], y = a[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { a[i] = y; a[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                        const uint32_t *__restrict__ cand_base,
                                                        uint32_t *__restrict__ nheavy)
 {
+    constexpr uint32_t kSel = 4096;   // candidates sorted by count (more are rare; the rest is dropped)
+    constexpr uint32_t kWaveKeys = 256;
     __shared__ uint32_t n, placed;
-    __shared__ uint32_t sel[kHeavyMaxKeys];
+    __shared__ uint64_t ord[kSel];                 // count << 32 | key
     __shared__ __align__(16) uint32_t tab[kHeavySlots];
     __shared__ uint32_t bfill[kHeavySlots / 4];   // slots taken (or asked for) per bucket
     const HeavyJob job = jobs[blockIdx.x];
@@ -750,33 +770,45 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     }
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) tab[i] = kEmptyKey;
     for (uint32_t i = threadIdx.x; i < kHeavySlots / 4; i += blockDim.x) bfill[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kSel; i += blockDim.x) ord[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kCandSlots; i += blockDim.x) {
         if (cand_keys[i] == kEmptyKey || cand_cnt[i] < min_count) continue;
         const uint32_t k = atomicAdd(&n, 1u);
-        if (k < kHeavyMaxKeys) sel[k] = cand_keys[i];
+        if (k < kSel) ord[k] = ((uint64_t)cand_cnt[i] << 32) | cand_keys[i];
     }
     __syncthreads();
-    // 2-choice placement, one key per thread: a slot of the emptier of the
-    // key's two 4-slot buckets (as seen now), else of the other; a key with
-    // both buckets full stays an ordinary key.  Keys are unique, so each
-    // sits in at most one slot.
-    static_assert(kHeavyMaxKeys <= 1024, "one selected key per thread");
-    if (threadIdx.x < min(n, kHeavyMaxKeys)) {
-        const uint32_t key = sel[threadIdx.x];
-        const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
-        const uint32_t first = bfill[b2] < bfill[b1] ? b2 : b1, second = first == b1 ? b2 : b1;
-        uint32_t slot = atomicAdd(&bfill[first], 1u), b = first;
-        if (slot >= 4 && second != first) {
-            slot = atomicAdd(&bfill[second], 1u);
-            b = second;
+    const uint32_t nsel = min(n, kSel);
+    if (nsel == 0) return;   // the table and count stay zero
+    // hottest first: ascending sort, then placement from the end in rounds
+    // of kWaveKeys keys, so that a key left out for two full buckets is
+    // one of the coldest selected (a hot key left ordinary can overflow a
+    // cell of the partition layout)
+    uint32_t m = 1;
+    while (m < nsel) m <<= 1;
+    bitonic_sort_lds(ord, m);   // ord[nsel, m) are zeros: the keys end up in ord[m - nsel, m)
+    const uint32_t nkeys = min(nsel, kHeavyMaxKeys);
+    // 2-choice placement, one key per thread per round: a slot of the
+    // emptier of the key's two 4-slot buckets (as seen now), else of the
+    // other; a key with both buckets full stays an ordinary key.  Keys are
+    // unique, so each sits in at most one slot.
+    for (uint32_t r0 = 0; r0 < nkeys; r0 += kWaveKeys) {
+        if (threadIdx.x < kWaveKeys && r0 + threadIdx.x < nkeys) {
+            const uint32_t key = (uint32_t)ord[m - 1 - (r0 + threadIdx.x)];
+            const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
+            const uint32_t first = bfill[b2] < bfill[b1] ? b2 : b1, second = first == b1 ? b2 : b1;
+            uint32_t slot = atomicAdd(&bfill[first], 1u), b = first;
+            if (slot >= 4 && second != first) {
+                slot = atomicAdd(&bfill[second], 1u);
+                b = second;
+            }
+            if (slot < 4) {
+                tab[4 * b + slot] = key;
+                atomicAdd(&placed, 1u);
+            }
         }
-        if (slot < 4) {
-            tab[4 * b + slot] = key;
-            atomicAdd(&placed, 1u);
-        }
+        __syncthreads();
     }
-    __syncthreads();
     for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
         gst32(I.heavy_keys, i, tab[i]);
         gst32(I.heavy_rids, i, kNoRid);
@@ -907,7 +939,7 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
 // that is recorded for the compaction step -- exact counts, no padding.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kJoinNT = 1024;
-constexpr uint32_t kJoinIPT = 8;
+constexpr uint32_t kJoinIPT = 4;   // S tuples per lane per probe chunk (smaller chunks: less end-of-item wait)
 constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (auto)
 constexpr uint32_t kBucketBits = 13;
 constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-bit ends: 16 KB)
@@ -969,23 +1001,6 @@ __device__ __forceinline__ void wave_emit_round(uint32_t hitm, const uint2 *p, J
             else atomicOr(err_i, kErrOutOverflow);
         }
         wb += (uint32_t)__popcll(m[j]);
-    }
-}
-
-__device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* power of two */)
-{
-    for (uint32_t k = 2; k <= n; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const uint64_t x = a[i], y = a[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((x > y) == up) { a[i] = y; a[ixj] = x; }
-                }
-            }
-            __syncthreads();
-        }
     }
 }
 
