@@ -404,6 +404,35 @@ def test_join_instance_plan_and_timeline(gpu, orc, replicate):
     ji.close()
 
 
+def test_join_instance_plan_errors_and_nphj(gpu, orc):
+    """A malformed plan is refused without disturbing the instance; a plain
+    plan runs join_alg 1 (no-partition join); the timeline of a join whose R
+    and S were scattered by one launch gives both partition nodes the same
+    interval."""
+    import test_plan
+
+    sj = gpu
+    R = sj.gen_r(1 << 15, 91)
+    S = sj.gen_s(1 << 17, R, 0.9, 92)
+    ji = sj.JoinInstance(16, 1, 16)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    for bad in ["{", test_plan.doc([test_plan.send("left")])]:
+        with pytest.raises(sj.SpjError) as e:
+            ji.set_query_plan(bad)
+        assert e.value.code == -1
+    ji.set_query_plan(test_plan.doc(test_plan.plain_tree()))
+    ji.start(sj.JoinInstance.design(join_alg=1))
+    check_equal(sj, orc, R, S, ji.finish())
+    assert ji.phase_times()["s_partitioned"] == 0
+    ji.start(sj.JoinInstance.design(join_alg=0))   # small relations: one scatter launch for R and S
+    check_equal(sj, orc, R, S, ji.finish())
+    nodes, _ = parse_timeline(ji.timeline(0))
+    parts = [n for n in nodes if n[5] == "COMPOUND_FUNC_GLB_PARTITION"]
+    assert len(parts) == 2 and parts[0][3:5] == parts[1][3:5]
+    ji.close()
+
+
 def test_batch_mixed_sizes(gpu, orc):
     """A batch where one instance has 256 partitions and heavy keys, next to
     small instances: tiles of different instances share scatter workgroups
