@@ -110,3 +110,27 @@ def test_invalid_plans_rejected(sj, tree, why):
         sj.plan_check(doc(tree))
     assert e.value.code == -1
     assert why in str(e.value), str(e.value)
+
+
+def header_plans():
+    """The join_json / bg_join_json macros of include/execution.hpp, as text."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                            "execution.hpp")).read()
+    out = {}
+    for name in ("join_json", "bg_join_json"):
+        body = re.search(r"#define %s \\\n((?:    \".*\"(?: \\)?\n)+)" % name, src).group(1)
+        pieces = re.findall(r'"((?:[^"\\]|\\.)*)"', body)
+        out[name] = "".join(p.replace('\\"', '"').replace("\\\\", "\\") for p in pieces)
+    return out
+
+
+def test_execution_header_plans(sj):
+    """include/execution.hpp's two plan strings are valid plans of the shape
+    StartJoin picks between."""
+    p = header_plans()
+    assert sj.plan_check(p["join_json"]) == (7, False)
+    assert sj.plan_check(p["bg_join_json"]) == (9, True)
+    assert json.loads(p["join_json"])["query_tree"] == plain_tree()
+    assert json.loads(p["bg_join_json"])["query_tree"] == bg_tree()
