@@ -446,13 +446,15 @@ def test_batch_mixed_sizes(gpu, orc):
         check_equal(sj, orc, R, S, p)
 
 
-@pytest.mark.parametrize("nR,nS,zipf", [(1 << 26, 1 << 27, 0.5), (1 << 26, 1 << 28, 1.5)])
-def test_full_size_checksum(gpu, orc, nR, nS, zipf):
+@pytest.mark.parametrize("nR,nS,zipf,alg", [(1 << 26, 1 << 27, 0.5, -1), (1 << 26, 1 << 28, 0.0, -1),
+                                              (1 << 26, 1 << 28, 1.0, -1), (1 << 26, 1 << 28, 1.5, -1),
+                                              (1 << 24, 1 << 26, 1.0, 1)])
+def test_full_size_checksum(gpu, orc, nR, nS, zipf, alg):
     """Full-size parity through a size-independent property: the pair count
     and the order-independent checksum of the GPU output (computed on the
     device) equal the oracle's closed-form checksum."""
     sj = gpu
-    print(f"case full nR={nR} nS={nS} zipf={zipf}", flush=True)
+    print(f"case full nR={nR} nS={nS} zipf={zipf} alg={alg}", flush=True)
     R = sj.gen_r(nR, 1000)
     S = sj.gen_s(nS, R, zipf, 1001)
     want = orc.join_checksum(R, S)
@@ -461,7 +463,7 @@ def test_full_size_checksum(gpu, orc, nR, nS, zipf):
     del S
     dO = sj.DeviceArray(nS * 8)
     try:
-        n, _ = sj.join_device(dR, nR, dS, nS, dO)
+        n, _ = sj.join_device(dR, nR, dS, nS, dO, sj.config(algorithm=alg) if alg >= 0 else None)
         got = (n,) + sj.checksum_device(dO.ptr, n)
     finally:
         dR.free()
