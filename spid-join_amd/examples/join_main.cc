@@ -1,7 +1,7 @@
-// join_main.cc -- the reference's test driver flow (test/join/main.cc:16-116)
+// join_main.cc -- the reference's join test program's flow (test/join/main.cc:16-116)
 // against this library's pidjoin::JoinInstance: same CLI flags
 //   -s input_size  -r ranks  -z zipf x100  -t R:S ratio  -R rank_set  -B bank_set
-// plus -a join_alg.  Unlike the reference driver it draws S keys Zipf over R
+// plus -a join_alg.  Unlike the reference's program it draws S keys Zipf over R
 // (the reference leaves Zipf as a stub, main.cc:45-48) and validates the
 // result: exact pair count and, for every pair, R.key[r_rid] == S.key[s_rid].
 #include <getopt.h>
@@ -53,7 +53,7 @@ int main(int argc, char **argv)
     ResultBuffers_t res = join_instance.FinishJoin();
     join_instance.RecordLogs(start, 0);
 
-    // validation: the reference's driver never reads its result (main.cc:63)
+    // validation: the reference's program never reads its result (main.cc:63)
     std::vector<uint32_t> key_of_rid((size_t)nR);
     for (int64_t i = 0; i < nR; i++) key_of_rid[r[(size_t)i].rvalue] = r[(size_t)i].lvalue;
     int64_t pairs = 0, bad = 0;

@@ -1,4 +1,4 @@
-"""GPU: the reference driver's JoinInstance flow (test/join/main.cc) through
+"""GPU: the reference join test program's JoinInstance flow (test/join/main.cc) through
 the C++ facade, built by __graft_entry__.build() as spid-join_amd/join_example."""
 import os
 import subprocess
@@ -12,7 +12,7 @@ EXE = os.path.join(ROOT, "spid-join_amd", "join_example")
 
 @pytest.mark.parametrize("args", [["-s", "1048576", "-t", "2"], ["-s", "262144", "-t", "8", "-z", "150", "-B", "8"],
                                   ["-s", "131072", "-t", "4", "-z", "50", "-a", "2"]])
-def test_reference_driver_flow(args):
+def test_reference_program_flow(args):
     if not os.path.exists(EXE):
         pytest.fail("join_example not built (run __graft_entry__.build())")
     print("case", args, flush=True)
