@@ -946,7 +946,6 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-b
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
-constexpr uint32_t kSortS = 2048;           // sort-merge: S chunk sorted per merge
 constexpr uint32_t kNLChunk = 8192;
 
 struct JoinShared {
@@ -1162,16 +1161,16 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 }
             }
         } else if (alg == 2) {
-            // ---------------- sort-merge (mpsm_join_sort_probe.c:152-236):
-            // the R partition is bitonic-sorted in LDS once; each S chunk is
-            // sorted and merged against it.  The merge of the sorted S chunk
-            // with the sorted R run is split among threads by merge path
-            // (a binary search on each thread's cross diagonal); a thread then
-            // merges its segment and emits one pair per S tuple whose key
-            // equals the current R key.
+            // ---------------- sort-merge (mpsm_join_sort_probe.c): the R
+            // partition is bitonic-sorted by key in LDS (runs of kSortR);
+            // each S tuple then takes the R entry at the lower bound of its
+            // key, found by binary search -- the entry the reference's merge
+            // of the two sorted runs (probe(), 246-272) pairs it with.  The
+            // merge walk needs S sorted; the search does not, so S is probed
+            // as it lies, IPT tuples per thread with their searches in
+            // lockstep.  Adjacent equal keys in a sorted run are duplicate R
+            // keys.
             uint64_t *ra = reinterpret_cast<uint64_t *>(lds);
-            uint64_t *sa = ra + kSortR;
-            uint32_t *cnts = reinterpret_cast<uint32_t *>(sa + kSortS);   // NT
             for (uint32_t c0 = 0; c0 < nr; c0 += kSortR) {
                 const uint32_t cn = min(kSortR, nr - c0);
                 uint32_t np = 1;
@@ -1182,69 +1181,49 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     else ra[i] = ~0ull;
                 }
                 __syncthreads();
-                bitonic_sort_lds(ra, np);
-                for (uint32_t s0 = w.sb; s0 < w.se; s0 += kSortS) {
-                    const uint32_t sn = min(kSortS, w.se - s0);
-                    uint32_t sp = 1;
-                    while (sp < sn) sp <<= 1;
-                    for (uint32_t i = threadIdx.x; i < sp; i += NT) {
-                        if (i < sn) { uint2 sv = gld(I.Sp, spos(s0 + i)); sa[i] = ((uint64_t)sv.x << 32) | sv.y; }
-                        else sa[i] = ~0ull;
+                bitonic_sort_lds(ra, np);   // ends with a barrier
+                {
+                    bool dup = false;
+                    for (uint32_t i = threadIdx.x; i + 1 < cn; i += NT) {
+                        const uint32_t k = (uint32_t)(ra[i] >> 32);
+                        dup |= k != kEmptyKey && k == (uint32_t)(ra[i + 1] >> 32);
                     }
-                    __syncthreads();
-                    bitonic_sort_lds(sa, sp);
-                    // merge path over A = S[0, sn), B = R[0, cn) ordered by key
-                    const uint32_t L = sn + cn, per = (L + NT - 1) / NT;
-                    const uint32_t d0 = min(L, threadIdx.x * per), d1 = min(L, d0 + per);
-                    uint32_t ai = 0, bi = 0, c = 0;
-                    {
-                        // a = number of S elements among the first d0 merged ones:
-                        // the first m whose S key exceeds R[d0-1-m] (S first on ties)
-                        uint32_t lo = d0 > cn ? d0 - cn : 0u, hi = min(d0, sn);
-                        while (lo < hi) {
-                            const uint32_t m = (lo + hi) >> 1;
-                            const uint32_t ka = (uint32_t)(sa[m] >> 32), kb = (uint32_t)(ra[d0 - 1 - m] >> 32);
-                            if (ka <= kb) lo = m + 1; else hi = m;
-                        }
-                        ai = lo;
-                        bi = d0 - lo;
+                    if (dup) atomicOr(erri, kErrDuplicateKey);
+                }
+                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                    uint2 v[IPT];
+                    load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
+                    uint32_t lo[IPT], hi[IPT];
+#pragma unroll
+                    for (int j = 0; j < (int)IPT; j++) {
+                        lo[j] = 0;
+                        hi[j] = cn;
                     }
-                    const uint32_t a_start = ai, b_start = bi;
-                    for (uint32_t d = d0; d < d1; d++) {
-                        // take from S when its key <= the R key: an S tuple is
-                        // consumed while the equal R key is still current
-                        const bool takeA = ai < sn && (bi >= cn || (uint32_t)(sa[ai] >> 32) <= (uint32_t)(ra[bi] >> 32));
-                        if (takeA) {
-                            const uint32_t k = (uint32_t)(sa[ai] >> 32);
-                            if (k != kEmptyKey && bi < cn && (uint32_t)(ra[bi] >> 32) == k) c++;
-                            ai++;
-                        } else {
-                            bi++;
-                        }
-                    }
-                    cnts[threadIdx.x] = c;
-                    __syncthreads();
-                    const uint32_t tot = block_exclusive_scan(cnts, NT, wtmp);
-                    uint32_t o = sh.fill + cnts[threadIdx.x];
-                    ai = a_start;
-                    bi = b_start;
-                    for (uint32_t d = d0; d < d1; d++) {
-                        const bool takeA = ai < sn && (bi >= cn || (uint32_t)(sa[ai] >> 32) <= (uint32_t)(ra[bi] >> 32));
-                        if (takeA) {
-                            const uint32_t k = (uint32_t)(sa[ai] >> 32);
-                            if (k != kEmptyKey && bi < cn && (uint32_t)(ra[bi] >> 32) == k) {
-                                if (o < sh.res) gst(I.out, sh.base + o, make_uint2((uint32_t)ra[bi], (uint32_t)sa[ai]));
-                                else atomicOr(erri, kErrOutOverflow);
-                                o++;
+                    // floor(log2 cn) + 1 halvings close every [lo, hi)
+                    for (uint32_t step = cn; step > 0; step >>= 1) {
+#pragma unroll
+                        for (int j = 0; j < (int)IPT; j++) {
+                            if (lo[j] < hi[j]) {
+                                const uint32_t m = (lo[j] + hi[j]) >> 1;
+                                if ((uint32_t)(ra[m] >> 32) < v[j].x) lo[j] = m + 1;
+                                else hi[j] = m;
                             }
-                            ai++;
-                        } else {
-                            bi++;
                         }
                     }
-                    __syncthreads();
-                    if (threadIdx.x == 0) sh.fill += tot;
-                    __syncthreads();
+                    uint32_t hitm = 0;
+                    uint2 pr[IPT];
+#pragma unroll
+                    for (int j = 0; j < (int)IPT; j++) {
+                        pr[j] = v[j];
+                        if (v[j].x != kEmptyKey && lo[j] < cn) {
+                            const uint64_t e = ra[lo[j]];
+                            if ((uint32_t)(e >> 32) == v[j].x) {
+                                hitm |= 1u << j;
+                                pr[j] = make_uint2((uint32_t)e, v[j].y);   // (R rid, S rid)
+                            }
+                        }
+                    }
+                    wave_emit_round<IPT>(hitm, pr, &sh, I.out, erri);
                 }
             }
         } else {
@@ -1661,7 +1640,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 size_t lds_join()
 {
     size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 2) * 2;
-    size_t smj = ((size_t)kSortR + kSortS) * 8 + kJoinNT * 4;
+    size_t smj = (size_t)kSortR * 8;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
     return m > nl ? m : nl;

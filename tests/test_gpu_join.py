@@ -126,6 +126,24 @@ def test_duplicate_r_key_is_an_error(gpu):
     assert e.value.code == -4
 
 
+@pytest.mark.parametrize("bits", [2, 6])
+def test_sort_merge_long_runs_and_duplicates(gpu, orc, bits):
+    """Sort-merge with R partitions longer than one sorted run (2^18 R tuples
+    over 4 or 64 partitions: several 16K runs each), and a duplicate R key
+    (adjacent in its sorted run) reported as an error."""
+    sj = gpu
+    R = sj.gen_r(1 << 18, 95 + bits)
+    S = sj.gen_s(1 << 19, R, 1.0, 96)
+    print(f"case smj bits={bits}", flush=True)
+    pairs, _ = sj.join(R, S, sj.config(algorithm=2, radix_bits=bits))
+    check_equal(sj, orc, R, S, pairs)
+    R2 = R.copy()
+    R2["key"][500] = R2["key"][9000]
+    with pytest.raises(sj.SpjError) as e:
+        sj.join(R2, S, sj.config(algorithm=2, radix_bits=8))
+    assert e.value.code == -4
+
+
 @pytest.mark.parametrize("nR,ratio,zipf", [(1 << 12, 2, 0.0), (1 << 16, 4, 1.0), (1 << 20, 2, 0.5),
                                            (100_003, 3, 1.5), (1, 5, 0.0), (1 << 21, 8, 0.0)])
 def test_no_partition_hash_join(gpu, orc, nR, ratio, zipf):
@@ -448,7 +466,7 @@ def test_batch_mixed_sizes(gpu, orc):
 
 @pytest.mark.parametrize("nR,nS,zipf,alg", [(1 << 26, 1 << 27, 0.5, -1), (1 << 26, 1 << 28, 0.0, -1),
                                               (1 << 26, 1 << 28, 1.0, -1), (1 << 26, 1 << 28, 1.5, -1),
-                                              (1 << 24, 1 << 26, 1.0, 1)])
+                                              (1 << 24, 1 << 26, 1.0, 1), (1 << 26, 1 << 27, 0.5, 2)])
 def test_full_size_checksum(gpu, orc, nR, nS, zipf, alg):
     """Full-size parity through a size-independent property: the pair count
     and the order-independent checksum of the GPU output (computed on the
