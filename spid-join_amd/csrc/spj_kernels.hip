@@ -747,6 +747,79 @@ __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* powe
     }
 }
 
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, uint32_t m)
+{
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, kWave), hi = __shfl_xor((uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The same network on a[0, 1024 E) with the elements in registers: element
+// e = t E + r is register r of thread t.  A stage whose partner distance j
+// is below E runs inside each thread, below 64 E across the lanes of a wave
+// (shuffles), and only the rest goes through LDS with a barrier.  For
+// 16K elements that is 10 barrier stages instead of 105.
+template <int E>
+__device__ void bitonic_sort_regs(uint64_t *a)
+{
+    constexpr uint32_t N = 1024u * E;
+    const uint32_t t = threadIdx.x, lane = lane_id();
+    uint64_t x[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) x[r] = a[t * E + r];
+    for (uint32_t k = 2; k <= N; k <<= 1) {
+        uint32_t j = k >> 1;
+        if (j >= 64u * E) {
+#pragma unroll
+            for (int r = 0; r < E; r++) a[t * E + r] = x[r];
+            __syncthreads();
+            for (; j >= 64u * E; j >>= 1) {
+                for (uint32_t p = t; p < N / 2; p += 1024u) {
+                    const uint32_t lo = ((p & ~(j - 1u)) << 1) | (p & (j - 1u)), hi = lo + j;
+                    const bool up = (lo & k) == 0;
+                    const uint64_t u = a[lo], v = a[hi];
+                    if ((u > v) == up) {
+                        a[lo] = v;
+                        a[hi] = u;
+                    }
+                }
+                __syncthreads();
+            }
+            // each thread reads back only its own elements
+#pragma unroll
+            for (int r = 0; r < E; r++) x[r] = a[t * E + r];
+        }
+        for (; j >= (uint32_t)E; j >>= 1) {
+            const uint32_t m = j / E;
+            const bool keep_min = ((lane & m) == 0) == (((t * E) & k) == 0);
+#pragma unroll
+            for (int r = 0; r < E; r++) {
+                const uint64_t y = shfl_xor64(x[r], m);
+                x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
+            }
+        }
+#pragma unroll
+        for (int jj = E / 2; jj >= 1; jj >>= 1) {
+            if ((uint32_t)jj < k) {
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    if ((r & jj) == 0) {
+                        const bool up = ((t * E + r) & k) == 0;
+                        const uint64_t u = x[r], v = x[r | jj];
+                        const bool sw = (u > v) == up;
+                        x[r] = sw ? v : u;
+                        x[r | jj] = sw ? u : v;
+                    }
+                }
+            }
+        }
+    }
+    // every thread has left the last LDS stage (its barrier) before any
+    // element is stored over
+#pragma unroll
+    for (int r = 0; r < E; r++) a[t * E + r] = x[r];
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                        const uint32_t *__restrict__ cand_base,
                                                        uint32_t *__restrict__ nheavy)
@@ -946,6 +1019,7 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-b
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
+static_assert(kJoinNT == 1024, "bitonic_sort_regs maps 1024 threads");
 constexpr uint32_t kNLChunk = 8192;
 
 struct JoinShared {
@@ -1181,7 +1255,11 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     else ra[i] = ~0ull;
                 }
                 __syncthreads();
-                bitonic_sort_lds(ra, np);   // ends with a barrier
+                // ends with a barrier
+                if (np == 16384) bitonic_sort_regs<16>(ra);
+                else if (np == 8192) bitonic_sort_regs<8>(ra);
+                else if (np == 4096) bitonic_sort_regs<4>(ra);
+                else bitonic_sort_lds(ra, np);
                 {
                     bool dup = false;
                     for (uint32_t i = threadIdx.x; i + 1 < cn; i += NT) {
