@@ -1019,6 +1019,7 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-b
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
+constexpr uint32_t kSortTail = 2048;        // sort-merge: a second, short run in the same pass
 static_assert(kJoinNT == 1024, "bitonic_sort_regs maps 1024 threads");
 constexpr uint32_t kNLChunk = 8192;
 
@@ -1244,27 +1245,57 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             // as it lies, IPT tuples per thread with their searches in
             // lockstep.  Adjacent equal keys in a sorted run are duplicate R
             // keys.
-            uint64_t *ra = reinterpret_cast<uint64_t *>(lds);
-            for (uint32_t c0 = 0; c0 < nr; c0 += kSortR) {
+            // A partition of up to kSortR + kSortTail tuples is one pass: a
+            // run of kSortR and a short second run (kSortR-tuple partitions
+            // overflow by a little about half the time; a second pass would
+            // read and search all of S again).
+            uint64_t *ra = reinterpret_cast<uint64_t *>(lds), *ra2 = ra + kSortR;
+            for (uint32_t c0 = 0; c0 < nr;) {
                 const uint32_t cn = min(kSortR, nr - c0);
-                uint32_t np = 1;
+                const uint32_t rest = nr - c0 - cn;
+                const uint32_t cn2 = rest <= kSortTail ? rest : 0u;
+                uint32_t np = 1, np2 = 1;
                 while (np < cn) np <<= 1;
+                while (np2 < cn2) np2 <<= 1;
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < np; i += NT) {
                     if (i < cn) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); ra[i] = ((uint64_t)r.x << 32) | r.y; }
                     else ra[i] = ~0ull;
                 }
+                if (cn2) {
+                    for (uint32_t i = threadIdx.x; i < np2; i += NT) {
+                        if (i < cn2) { uint2 r = gld(I.Rp, rpos(rb + c0 + cn + i)); ra2[i] = ((uint64_t)r.x << 32) | r.y; }
+                        else ra2[i] = ~0ull;
+                    }
+                }
                 __syncthreads();
-                // ends with a barrier
+                // each ends with a barrier
                 if (np == 16384) bitonic_sort_regs<16>(ra);
                 else if (np == 8192) bitonic_sort_regs<8>(ra);
                 else if (np == 4096) bitonic_sort_regs<4>(ra);
                 else bitonic_sort_lds(ra, np);
+                if (cn2) bitonic_sort_lds(ra2, np2);
+                // lower bound of key k in the sorted run x[0, n)
+                auto lower = [](const uint64_t *x, uint32_t n, uint32_t k) {
+                    uint32_t lo = 0, hi = n;
+                    while (lo < hi) {
+                        const uint32_t m = (lo + hi) >> 1;
+                        if ((uint32_t)(x[m] >> 32) < k) lo = m + 1; else hi = m;
+                    }
+                    return lo;
+                };
                 {
                     bool dup = false;
                     for (uint32_t i = threadIdx.x; i + 1 < cn; i += NT) {
                         const uint32_t k = (uint32_t)(ra[i] >> 32);
                         dup |= k != kEmptyKey && k == (uint32_t)(ra[i + 1] >> 32);
+                    }
+                    for (uint32_t i = threadIdx.x; i < cn2; i += NT) {   // the second run: itself and against the first
+                        const uint32_t k = (uint32_t)(ra2[i] >> 32);
+                        if (k == kEmptyKey) continue;
+                        dup |= i + 1 < cn2 && k == (uint32_t)(ra2[i + 1] >> 32);
+                        const uint32_t q = lower(ra, cn, k);
+                        dup |= q < cn && (uint32_t)(ra[q] >> 32) == k;
                     }
                     if (dup) atomicOr(erri, kErrDuplicateKey);
                 }
@@ -1301,8 +1332,20 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             }
                         }
                     }
+                    if (cn2) {   // keys not in the first run: the second
+#pragma unroll
+                        for (int j = 0; j < (int)IPT; j++) {
+                            if ((hitm >> j) & 1u || v[j].x == kEmptyKey) continue;
+                            const uint32_t q = lower(ra2, cn2, v[j].x);
+                            if (q < cn2 && (uint32_t)(ra2[q] >> 32) == v[j].x) {
+                                hitm |= 1u << j;
+                                pr[j] = make_uint2((uint32_t)ra2[q], v[j].y);
+                            }
+                        }
+                    }
                     wave_emit_round<IPT>(hitm, pr, &sh, I.out, erri);
                 }
+                c0 += cn + cn2;
             }
         } else {
             // ---------------- hash join
@@ -1718,7 +1761,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 size_t lds_join()
 {
     size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 2) * 2;
-    size_t smj = (size_t)kSortR * 8;
+    size_t smj = ((size_t)kSortR + kSortTail) * 8;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
     return m > nl ? m : nl;

@@ -126,11 +126,12 @@ def test_duplicate_r_key_is_an_error(gpu):
     assert e.value.code == -4
 
 
-@pytest.mark.parametrize("bits", [2, 6])
+@pytest.mark.parametrize("bits", [2, 4, 6])
 def test_sort_merge_long_runs_and_duplicates(gpu, orc, bits):
     """Sort-merge with R partitions longer than one sorted run (2^18 R tuples
-    over 4 or 64 partitions: several 16K runs each), and a duplicate R key
-    (adjacent in its sorted run) reported as an error."""
+    over 4 partitions: four 16K runs each), around one run (16 partitions of
+    16K +- 128: about half add a short second run to the same pass) and
+    below it (64), and a duplicate R key reported as an error."""
     sj = gpu
     R = sj.gen_r(1 << 18, 95 + bits)
     S = sj.gen_s(1 << 19, R, 1.0, 96)
