@@ -1621,6 +1621,12 @@ __device__ __forceinline__ uint32_t nphj_job_of(const NphjJob *jobs, uint32_t nj
     return lo;
 }
 
+__global__ __launch_bounds__(256) void k_nphj_clear(unsigned long long *__restrict__ t, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        t[i] = 0ull;
+}
+
 __global__ __launch_bounds__(256) void k_nphj_build(const InstDesc *__restrict__ insts, const NphjJob *__restrict__ jobs,
                                                     uint32_t njobs, unsigned long long *__restrict__ tables,
                                                     uint32_t *__restrict__ err)
@@ -1893,9 +1899,11 @@ hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njob
                        uint32_t *out_fill, uint32_t *err, hipStream_t st)
 {
     if (!njobs) return hipSuccess;
-    hipError_t e = hipMemsetAsync(tables, 0, table_words * 8, st);
-    if (e != hipSuccess) return e;
+    // a fill kernel rather than a memset node, so that the sequence stays
+    // capturable into a graph
     auto *t = reinterpret_cast<unsigned long long *>(tables);
+    const uint64_t fb = std::min<uint64_t>((table_words + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_nphj_clear, dim3((uint32_t)fb), dim3(256), 0, st, t, table_words);
     if (build_blocks)
         hipLaunchKernelGGL(k_nphj_build, dim3(build_blocks), dim3(256), 0, st, insts, jobs, njobs, t, err);
     if (probe_blocks)
