@@ -387,9 +387,18 @@ def main():
     kname, roof, kernels, b_alg = roofline(res)
     ms = res["ms_per_step"]
     value = (nR + nS) / (ms * 1e-3)
+    b_min = 8 * (nR + nS) + 8 * res["pairs"]   # compulsory: read R and S once, write the pairs
+    measured = {}
+    for k in ("k_scatter", "k_join"):   # rocprof HBM-side bytes per join over the phase's HIP-event time
+        tb = measured_traffic(k)
+        if tb is not None and kernels[k]["ms"] > 0:
+            measured[k] = {"bytes": tb, "GB/s": round(tb / (kernels[k]["ms"] * 1e-3) / 1e9, 1)}
     extra = {"kernels": kernels, "dominant_kernel": kname,
              "join_B_alg_bytes": b_alg,
              "join_roofline_frac": round(b_alg / (ms * 1e-3) / HBM_PEAK, 4),
+             "join_B_min_bytes": b_min,
+             "join_B_min_frac": round(b_min / (ms * 1e-3) / HBM_PEAK, 4),
+             "measured_traffic": measured,
              "phases_ms": {k: round(v, 4) for k, v in res["phases"].items()}}
     if args.extra:
         smj = run_workload(nR, nS, zipf, max(3, args.steps // 4), 2, sj.config(algorithm=sj.ALG_SMJ), 1000, keep)
