@@ -747,10 +747,29 @@ __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* powe
     }
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, uint32_t m)
+// Lane exchange with lane ^ M: DPP for masks 1 and 2 (quad permutes) and 8
+// (rotating a 16-lane row by 8), a swizzle for the other masks below 32, a
+// permute through the LDS crossbar for 32.
+template <uint32_t M>
+__device__ __forceinline__ uint32_t xlane(uint32_t v)
 {
-    const uint32_t lo = __shfl_xor((uint32_t)v, m, kWave), hi = __shfl_xor((uint32_t)(v >> 32), m, kWave);
-    return ((uint64_t)hi << 32) | lo;
+    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    else if constexpr (M == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));   // xor M in 32 lanes
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane_id() ^ M) << 2), (int)v);
+}
+
+template <uint32_t M, int E>
+__device__ __forceinline__ void xlane_stage(uint64_t *x, uint32_t k, uint32_t lane, bool up)
+{
+    if (M * (uint32_t)E > (k >> 1)) return;   // no stage j = M E in this merge
+    const bool keep_min = ((lane & M) == 0) == up;
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const uint64_t y = ((uint64_t)xlane<M>((uint32_t)(x[r] >> 32)) << 32) | xlane<M>((uint32_t)x[r]);
+        x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
+    }
 }
 
 // The same network on a[0, 1024 E) with the elements in registers: element
@@ -788,15 +807,15 @@ __device__ void bitonic_sort_regs(uint64_t *a)
 #pragma unroll
             for (int r = 0; r < E; r++) x[r] = a[t * E + r];
         }
-        for (; j >= (uint32_t)E; j >>= 1) {
-            const uint32_t m = j / E;
-            const bool keep_min = ((lane & m) == 0) == (((t * E) & k) == 0);
-#pragma unroll
-            for (int r = 0; r < E; r++) {
-                const uint64_t y = shfl_xor64(x[r], m);
-                x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
-            }
-        }
+        // partner lane ^ M for the stage j = M E (largest first), each only
+        // when j <= k / 2
+        const bool up = ((t * E) & k) == 0;   // k > j >= E: the same for every r
+        xlane_stage<32, E>(x, k, lane, up);
+        xlane_stage<16, E>(x, k, lane, up);
+        xlane_stage<8, E>(x, k, lane, up);
+        xlane_stage<4, E>(x, k, lane, up);
+        xlane_stage<2, E>(x, k, lane, up);
+        xlane_stage<1, E>(x, k, lane, up);
 #pragma unroll
         for (int jj = E / 2; jj >= 1; jj >>= 1) {
             if ((uint32_t)jj < k) {
