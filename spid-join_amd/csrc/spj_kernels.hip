@@ -775,37 +775,43 @@ __device__ __forceinline__ void xlane_stage(uint64_t *x, uint32_t k, uint32_t la
 // The same network on a[0, 1024 E) with the elements in registers: element
 // e = t E + r is register r of thread t.  A stage whose partner distance j
 // is below E runs inside each thread, below 64 E across the lanes of a wave
-// (shuffles), and only the rest goes through LDS with a barrier.  For
-// 16K elements that is 10 barrier stages instead of 105.
+// (lane exchanges), and only the rest goes through LDS with a barrier: for
+// 16K elements 10 barrier stages instead of 105.  Between those stages the
+// elements sit in LDS at e + e / 16 (a pad word every 16): a thread's run
+// of E then starts 17 words after its neighbour's instead of 16, which
+// keeps the lanes of a wave off the same banks.  a must hold 17/16 of the
+// elements; they are read and left in a[0, 1024 E) as given.
 template <int E>
 __device__ void bitonic_sort_regs(uint64_t *a)
 {
     constexpr uint32_t N = 1024u * E;
     const uint32_t t = threadIdx.x, lane = lane_id();
+    auto P = [](uint32_t e) { return e + (e >> 4); };
     uint64_t x[E];
 #pragma unroll
     for (int r = 0; r < E; r++) x[r] = a[t * E + r];
+    __syncthreads();   // every element is read before the padded layout overwrites them
     for (uint32_t k = 2; k <= N; k <<= 1) {
         uint32_t j = k >> 1;
         if (j >= 64u * E) {
 #pragma unroll
-            for (int r = 0; r < E; r++) a[t * E + r] = x[r];
+            for (int r = 0; r < E; r++) a[P(t * E + r)] = x[r];
             __syncthreads();
             for (; j >= 64u * E; j >>= 1) {
                 for (uint32_t p = t; p < N / 2; p += 1024u) {
                     const uint32_t lo = ((p & ~(j - 1u)) << 1) | (p & (j - 1u)), hi = lo + j;
                     const bool up = (lo & k) == 0;
-                    const uint64_t u = a[lo], v = a[hi];
+                    const uint64_t u = a[P(lo)], v = a[P(hi)];
                     if ((u > v) == up) {
-                        a[lo] = v;
-                        a[hi] = u;
+                        a[P(lo)] = v;
+                        a[P(hi)] = u;
                     }
                 }
                 __syncthreads();
             }
             // each thread reads back only its own elements
 #pragma unroll
-            for (int r = 0; r < E; r++) x[r] = a[t * E + r];
+            for (int r = 0; r < E; r++) x[r] = a[P(t * E + r)];
         }
         // partner lane ^ M for the stage j = M E (largest first), each only
         // when j <= k / 2
@@ -822,9 +828,9 @@ __device__ void bitonic_sort_regs(uint64_t *a)
 #pragma unroll
                 for (int r = 0; r < E; r++) {
                     if ((r & jj) == 0) {
-                        const bool up = ((t * E + r) & k) == 0;
+                        const bool upr = ((t * E + r) & k) == 0;
                         const uint64_t u = x[r], v = x[r | jj];
-                        const bool sw = (u > v) == up;
+                        const bool sw = (u > v) == upr;
                         x[r] = sw ? v : u;
                         x[r | jj] = sw ? u : v;
                     }
@@ -832,8 +838,7 @@ __device__ void bitonic_sort_regs(uint64_t *a)
             }
         }
     }
-    // every thread has left the last LDS stage (its barrier) before any
-    // element is stored over
+    __syncthreads();   // every padded element is read back before the plain layout overwrites them
 #pragma unroll
     for (int r = 0; r < E; r++) a[t * E + r] = x[r];
     __syncthreads();
@@ -1038,7 +1043,8 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-b
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
-constexpr uint32_t kSortTail = 2048;        // sort-merge: a second, short run in the same pass
+constexpr uint32_t kSortTail = 1024;        // sort-merge: a second, short run in the same pass
+constexpr uint32_t kSortPad = kSortR + kSortR / 16;   // the register sort's padded layout
 static_assert(kJoinNT == 1024, "bitonic_sort_regs maps 1024 threads");
 constexpr uint32_t kNLChunk = 8192;
 
@@ -1268,7 +1274,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
             // run of kSortR and a short second run (kSortR-tuple partitions
             // overflow by a little about half the time; a second pass would
             // read and search all of S again).
-            uint64_t *ra = reinterpret_cast<uint64_t *>(lds), *ra2 = ra + kSortR;
+            uint64_t *ra = reinterpret_cast<uint64_t *>(lds), *ra2 = ra + kSortPad;
             for (uint32_t c0 = 0; c0 < nr;) {
                 const uint32_t cn = min(kSortR, nr - c0);
                 const uint32_t rest = nr - c0 - cn;
@@ -1786,7 +1792,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 size_t lds_join()
 {
     size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 2) * 2;
-    size_t smj = ((size_t)kSortR + kSortTail) * 8;
+    size_t smj = ((size_t)kSortPad + kSortTail) * 8;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
     return m > nl ? m : nl;
