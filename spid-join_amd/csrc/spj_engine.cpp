@@ -400,6 +400,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         j.inst = (uint32_t)i;
         j.log2cap = 10;
         while ((1ull << j.log2cap) < 2ull * specs[i].nR) j.log2cap++;
+        if (j.log2cap > 31) return hipErrorInvalidValue;   // the table's slot index is 32-bit: |R| <= 2^30
         j.table = nphj_words;
         j.first_build = (uint32_t)nphj_bb;
         j.first_probe = (uint32_t)nphj_pb;
