@@ -1498,15 +1498,24 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
+                    // the next kScan entries are read unconditionally (the
+                    // table has kScan slack slots past its last entry), so
+                    // that the reads overlap; only buckets longer than that
+                    // finish in a loop
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = u * NT + threadIdx.x;
-                        if (i + 1 < ee[u]) dup |= (tkv[i + 1].x == kk[u]);
+                        if (i >= ntab) continue;
+#pragma unroll
+                        for (uint32_t d = 1; d <= kScan; d++) {
+                            const uint32_t kd = tkv[i + d].x;   // i + d < ntab + kScan: inside the table
+                            dup |= (i + d < ee[u]) & (kd == kk[u]);
+                        }
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < RPT; u++) {
                         const uint32_t i = u * NT + threadIdx.x;
-                        for (uint32_t j = i + 2; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
+                        for (uint32_t j = i + kScan + 1; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
                     }
                     if (dup) atomicOr(erri, kErrDuplicateKey);
                 }
