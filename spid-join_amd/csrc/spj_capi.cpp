@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -61,6 +62,24 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
     if (cfg->part_layout >= 0) r.cells = (uint32_t)cfg->part_layout;
     if (cfg->cell_capacity_pct >= 0) r.cell_pct = (uint32_t)cfg->cell_capacity_pct;
     return r;
+}
+
+// AUTO picks the no-partition join for a call whose R relations total at
+// most 2^20 tuples, unless a partition parameter is set.  Its tables (16 B
+// per R tuple) then stay in the caches and there is no partition pass: on
+// MI355X it took 0.07-0.19 ms where the partitioned join took 0.09-0.35 ms
+// (|R| = 2^12..2^20, |S| = 4|R|), and it lost from |R| = 2^22 on.  A batch
+// is judged by its total, since the tables of a large batch do not stay
+// cached.
+constexpr uint64_t kAutoNphjMaxR = uint64_t(1) << 20;
+
+bool auto_no_partition(const spj_config_t *cfg, uint64_t total_r)
+{
+    if (total_r > kAutoNphjMaxR) return false;
+    if (!cfg) return true;
+    return cfg->algorithm < 0 && cfg->radix_bits < 0 && cfg->table_log2 < 0 && cfg->heavy < 0 &&
+           cfg->heavy_min_count < 0 && cfg->scatter_ipt < 0 && cfg->tile_tuples <= 0 && cfg->items_s_max <= 0 &&
+           cfg->part_layout < 0 && cfg->cell_capacity_pct < 0;
 }
 
 int check_resolved(const Resolved &r)
@@ -241,6 +260,7 @@ int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uin
     s.nR = (uint32_t)nR;
     s.nS = (uint32_t)nS;
     s.cfg = resolve(cfg, nR, nS, -1.0);
+    if (auto_no_partition(cfg, nR)) s.cfg.alg = SPJ_ALG_NPHJ;
     rc = check_resolved(s.cfg);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(default_context_mutex());
@@ -296,6 +316,9 @@ int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR
 {
     if (n < 0 || (n > 0 && (!R || !nR || !S || !nS || !out || !out_count))) return fail(SPJ_EINVAL, "null argument");
     std::vector<InstSpec> specs((size_t)n);
+    uint64_t total_r = 0;
+    for (int i = 0; i < n; i++) total_r += std::min<uint64_t>(nR[i], kAutoNphjMaxR + 1);   // no wrap-around
+    const bool nphj = auto_no_partition(cfg, total_r);
     for (int i = 0; i < n; i++) {
         int rc = validate_sizes(nR[i], nS[i]);
         if (rc) return rc;
@@ -306,6 +329,7 @@ int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR
         s.nR = (uint32_t)nR[i];
         s.nS = (uint32_t)nS[i];
         s.cfg = resolve(cfg, nR[i], nS[i], -1.0);
+        if (nphj) s.cfg.alg = SPJ_ALG_NPHJ;
         rc = check_resolved(s.cfg);
         if (rc) return rc;
     }

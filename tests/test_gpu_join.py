@@ -26,7 +26,7 @@ def test_hash_join_matches_oracle(gpu, orc, nR, ratio, zipf):
     R = sj.gen_r(nR, 100 + nR)
     S = sj.gen_s(nR * ratio, R, zipf, 200 + nR)
     print(f"case nR={nR} nS={len(S)} zipf={zipf}", flush=True)
-    pairs, t = sj.join(R, S, timed=True)
+    pairs, t = sj.join(R, S, sj.config(algorithm=0), timed=True)
     print(f"  times={t}", flush=True)
     check_equal(sj, orc, R, S, pairs)
 
@@ -73,7 +73,7 @@ def test_cell_overflow_redone_counted(gpu, orc, pct):
     pairs, t = sj.join(R, S, sj.config(part_layout=1, cell_capacity_pct=pct), timed=True)
     check_equal(sj, orc, R, S, pairs)
     assert t["s_cells"] == 0 and t["r_cells"] == 0   # redone counted
-    pairs, t = sj.join(R, S, timed=True)   # automatic capacity: no overflow
+    pairs, t = sj.join(R, S, sj.config(algorithm=0), timed=True)   # automatic capacity: no overflow
     check_equal(sj, orc, R, S, pairs)
     assert t["s_cells"] == 1 and t["r_cells"] == 0   # R below 2^20 tuples: counted
     pairs, t = sj.join(R, S, sj.config(heavy=0), timed=True)   # no heavy handling: S stays counted
@@ -107,13 +107,14 @@ def test_tiny_and_empty(gpu, orc):
     sj = gpu
     R = sj.gen_r(5, 1)
     S = sj.gen_s(17, R, 0.0, 2)
-    pairs, _ = sj.join(R, S)
-    check_equal(sj, orc, R, S, pairs)
     E = np.zeros(0, dtype=sj.TUPLE)
-    pairs, _ = sj.join(E, S)
-    assert len(pairs) == 0
-    pairs, _ = sj.join(R, E)
-    assert len(pairs) == 0
+    for cfg in (None, sj.config(algorithm=0)):
+        pairs, _ = sj.join(R, S, cfg)
+        check_equal(sj, orc, R, S, pairs)
+        pairs, _ = sj.join(E, S, cfg)
+        assert len(pairs) == 0
+        pairs, _ = sj.join(R, E, cfg)
+        assert len(pairs) == 0
 
 
 def test_duplicate_r_key_is_an_error(gpu):
@@ -121,9 +122,10 @@ def test_duplicate_r_key_is_an_error(gpu):
     R = sj.gen_r(1 << 12, 3)
     R["key"][10] = R["key"][11]
     S = sj.gen_s(1 << 13, sj.gen_r(1 << 12, 3), 0.0, 4)
-    with pytest.raises(sj.SpjError) as e:
-        sj.join(R, S)
-    assert e.value.code == -4
+    for cfg in (None, sj.config(algorithm=0)):
+        with pytest.raises(sj.SpjError) as e:
+            sj.join(R, S, cfg)
+        assert e.value.code == -4
 
 
 @pytest.mark.parametrize("bits", [2, 4, 6])
@@ -207,6 +209,35 @@ def test_no_partition_hash_join_batch_and_instance(gpu, orc):
     ji.close()
 
 
+def test_auto_picks_no_partition_join_for_small_calls(gpu, orc):
+    """AUTO runs the no-partition join (no partition pass: r_partitioned 0)
+    when a call's R relations total at most 2^20 tuples and no partition
+    parameter is set; otherwise the partitioned join."""
+    sj = gpu
+    R = sj.gen_r(1 << 20, 171)
+    S = sj.gen_s(1 << 21, R, 1.2, 172)
+    print("case auto single", flush=True)
+    for cfg, parted in ((None, False), (sj.config(), False), (sj.config(radix_bits=9), True),
+                        (sj.config(heavy=1), True), (sj.config(algorithm=0), True)):
+        pairs, t = sj.join(R, S, cfg, timed=True)
+        check_equal(sj, orc, R, S, pairs)
+        assert (t["r_partitioned"] == len(R)) == parted, (cfg, t)
+    R2 = sj.gen_r((1 << 20) + 1, 173)
+    S2 = sj.gen_s(1 << 21, R2, 0.5, 174)
+    pairs, t = sj.join(R2, S2, timed=True)
+    check_equal(sj, orc, R2, S2, pairs)
+    assert t["r_partitioned"] == len(R2)
+    # a batch is judged by its total |R|
+    Rs = [sj.gen_r(1 << 18, 180 + i) for i in range(5)]
+    Ss = [sj.gen_s(1 << 19, Rs[i], 0.8, 190 + i) for i in range(5)]
+    print("case auto batch", flush=True)
+    for k, parted in ((4, False), (5, True)):
+        outs, t = sj.join_batch(Rs[:k], Ss[:k], timed=True)
+        for R_, S_, p in zip(Rs[:k], Ss[:k], outs):
+            check_equal(sj, orc, R_, S_, p)
+        assert (t["r_partitioned"] > 0) == parted, (k, t)
+
+
 @pytest.mark.parametrize("bits", [0, 3, 9, 12])
 def test_radix_bits_sweep(gpu, orc, bits):
     """Any radix fan-out gives the same multiset (large partitions are split
@@ -229,7 +260,7 @@ def test_duplicate_r_key_in_large_partition(gpu, copies):
     R["key"][1000:1000 + copies] = R["key"][77]
     S = sj.gen_s(1 << 19, sj.gen_r(1 << 18, 11 + copies), 0.0, 12)
     with pytest.raises(sj.SpjError) as e:
-        sj.join(R, S)
+        sj.join(R, S, sj.config(algorithm=0))
     assert e.value.code == -4
 
 
@@ -244,7 +275,7 @@ def test_batch_matches_single(gpu, orc):
         S = sj.gen_s(nR * ratio, R, float(rng.uniform(0, 1.25)), 2000 + i)
         Rs.append(R)
         Ss.append(S)
-    outs, _ = sj.join_batch(Rs, Ss)
+    outs, _ = sj.join_batch(Rs, Ss, sj.config(algorithm=0))
     for R, S, p in zip(Rs, Ss, outs):
         check_equal(sj, orc, R, S, p)
 
@@ -266,7 +297,7 @@ def test_batch_many_tiny_and_empty(gpu, orc):
     Rs.append(sj.gen_r(1 << 18, 5000))
     Ss.append(sj.gen_s(1 << 21, Rs[-1], 1.5, 5001))
     print("case batch many tiny", flush=True)
-    outs, _ = sj.join_batch(Rs, Ss)
+    outs, _ = sj.join_batch(Rs, Ss, sj.config(algorithm=0))
     for R, S, p in zip(Rs, Ss, outs):
         check_equal(sj, orc, R, S, p)
 
