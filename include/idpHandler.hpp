@@ -1,5 +1,6 @@
 // idpHandler.hpp -- only the run configuration of the reference's
-// idpHandler.hpp (join_design_t, 131-145, and the algorithm ids, 126-129).
+// idpHandler.hpp: pidjoin::join_design_t (131-145, inside namespace pidjoin
+// as there, 28) and the algorithm ids (126-129).
 // The UPMEM device handle, MRAM allocator and argument handlers have no GPU
 // counterpart.
 #pragma once
@@ -12,6 +13,8 @@
 #define JOIN_DESIGN_JOIN_ALGORITHM_NPHJ 1
 #define JOIN_DESIGN_JOIN_ALGORITHM_SMJ 2
 #define JOIN_DESIGN_JOIN_ALGORITHM_NL 3
+
+namespace pidjoin {
 
 typedef struct join_design_t {
     int64_t input_size = 500000;
@@ -26,3 +29,5 @@ typedef struct join_design_t {
     std::string foreign_key;
     int identical_value_percent = 0;
 } join_design_t;
+
+}  // namespace pidjoin

@@ -2,6 +2,7 @@
 // API (join_internals.hpp:301-323, src/query.cc), on one MI355X.
 //
 //   JoinInstance(ranks, rank_groups, ranks_per_group)   query.cc:842
+//   JoinInstance(ranks, rank_group_id)  -- deprecated    query.cc:733
 //   LoadColumn(ptr, n, "left" | "right")                query.cc:213
 //   PrepareJoin(design)                                  query.cc:178
 //   StartJoin(design)   -- asynchronous                  query.cc:188
@@ -28,6 +29,9 @@ class JoinInstance {
 public:
     JoinInstance();
     explicit JoinInstance(int num_rank_allocated_);
+    // Deprecated in the reference (join_internals.hpp:305), whose body prints
+    // an error and exits (query.cc:733-736); here it throws std::logic_error.
+    [[deprecated]] JoinInstance(int num_rank_allocated_, int rank_group_id);
     JoinInstance(int num_rank_allocated_, int num_rankgroup, int num_rank_in_rankgroup);
     ~JoinInstance();
     JoinInstance(const JoinInstance &) = delete;

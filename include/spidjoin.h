@@ -37,7 +37,7 @@ enum {
     SPJ_EHIP = -2,         /* HIP runtime error */
     SPJ_ENOMEM = -3,       /* device or host allocation failed */
     SPJ_EDUPKEY = -4,      /* duplicate R key (a2a_srj_build.c:265-271) */
-    SPJ_EOVERFLOW = -5,    /* LDS table or output capacity exceeded */
+    SPJ_EOVERFLOW = -5,    /* output capacity exceeded */
     SPJ_ESTATE = -6,       /* call out of order (e.g. FinishJoin before StartJoin) */
     SPJ_EBUSY = -7         /* admission refused (serving HBM budget) */
 };
@@ -107,6 +107,16 @@ int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum
  * out must hold out_cap >= nS pairs.  times may be NULL. */
 int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
                     uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg, spj_phase_times_t *times);
+
+/* The global partition pass alone, on one device-resident relation
+ * (COMPOUND_FUNC_GLB_PARTITION, queryCompoundFunc.cc:814-872: GLB_PARTITION_COUNT,
+ * the page histogram, GLB_PARTITION_PACKET).  Tuple t goes to partition
+ * glb_partition_hash(t.key) & (2^radix_bits - 1) (glb_partition_count.c:192-195);
+ * key-0 tuples are skipped.  out (device, n tuples) receives the partitions
+ * back to back, partition p at [pstart[p], pstart[p + 1]) in no particular
+ * order inside; pstart (device) receives 2^radix_bits + 1 offsets, the last
+ * one the number of non-zero keys.  radix_bits 0..12. */
+int spj_partition_device(const spj_tuple_t *rel, uint64_t n, int radix_bits, spj_tuple_t *out, uint32_t *pstart);
 
 /* Same on host buffers (H2D, join, D2H of exactly out_count pairs). */
 int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,

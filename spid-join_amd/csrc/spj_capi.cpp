@@ -42,7 +42,6 @@ int hip_fail(hipError_t e, const char *what)
 int err_from_flags(uint32_t f, size_t inst)
 {
     if (f & kErrDuplicateKey) return fail(SPJ_EDUPKEY, "instance %zu: duplicate key on the R (build) side", inst);
-    if (f & kErrTableOverflow) return fail(SPJ_EOVERFLOW, "instance %zu: a partition overflowed its LDS hash table", inst);
     if (f & kErrOutOverflow) return fail(SPJ_EOVERFLOW, "instance %zu: output capacity exceeded", inst);
     return SPJ_OK;
 }
@@ -271,6 +270,34 @@ int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uin
     fill_times(t, times);
     *out_count = counts[0];
     return err_from_flags(errs[0], 0);
+}
+
+int spj_partition_device(const spj_tuple_t *rel, uint64_t n, int radix_bits, spj_tuple_t *out, uint32_t *pstart)
+{
+    if ((!rel && n) || (!out && n) || !pstart) return fail(SPJ_EINVAL, "null argument");
+    if (radix_bits < 0 || radix_bits > (int)kMaxBits) return fail(SPJ_EINVAL, "radix_bits %d outside 0..%u", radix_bits, kMaxBits);
+    int rc = validate_sizes(n, 0);
+    if (rc) return rc;
+    // an instance with an empty S whose R goes through the counted layout
+    // (histogram, scan, scatter) into contiguous partitions
+    InstSpec s;
+    s.R = reinterpret_cast<const uint2 *>(rel);
+    s.nR = (uint32_t)n;
+    s.S = s.R;
+    s.nS = 0;
+    s.out = reinterpret_cast<uint2 *>(out);
+    s.cfg = resolve(nullptr, n, n, -1.0);
+    s.cfg.bits = (uint32_t)radix_bits;
+    s.cfg.alg = 0;
+    s.cfg.heavy = 0;
+    s.cfg.cells = 0;
+    s.r_contiguous = true;
+    std::lock_guard<std::mutex> lk(default_context_mutex());
+    Context &ctx = default_context();
+    std::vector<uint32_t> counts, errs;
+    hipError_t e = ctx.run({s}, counts, errs, nullptr);
+    if (e == hipSuccess) e = ctx.export_partitioned_r(0, reinterpret_cast<uint2 *>(out), pstart);
+    return e == hipSuccess ? SPJ_OK : hip_fail(e, "partition");
 }
 
 int spj_join_host(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,

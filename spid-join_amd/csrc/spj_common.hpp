@@ -27,8 +27,13 @@ namespace spj {
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kEmptyKey = 0u;          // key 0 = empty / skipped (glb_partition_count.c:192)
-constexpr uint32_t kNoRid = 0xFFFFFFFFu;    // heavy key with no R tuple yet
 constexpr uint32_t kHeavySlots = 4096;      // heavy (key -> R rid) table, LDS resident
+// One instance's heavy table in device memory: kHeavySlots keys (0 = empty),
+// kHeavySlots R rids, then a presence bitmap (bit s set: slot s's key has an
+// R tuple and its rid is valid).  Presence is kept apart from the rid because
+// every 32-bit value is a legal rid (kv_pair_t.rvalue is a full uint32).
+constexpr uint32_t kHeavyPresWords = kHeavySlots / 32;
+constexpr uint32_t kHeavyWords = 2 * kHeavySlots + kHeavyPresWords;
 constexpr uint32_t kHeavyMaxKeys = 1024;    // at most a quarter full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
 constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
@@ -37,7 +42,6 @@ constexpr uint32_t kSinkTuples = kScatterNT * 12;   // largest scatter round
 
 // error bits in err[inst]
 constexpr uint32_t kErrDuplicateKey = 1u;   // a2a_srj_build.c:265-271 (VALIDATION)
-constexpr uint32_t kErrTableOverflow = 2u;  // a sub-table received more tuples than slots
 constexpr uint32_t kErrOutOverflow = 4u;    // more pairs than the output capacity
 constexpr uint32_t kErrCellOverflow = 8u;   // an S cell received more tuples than its capacity
 
@@ -53,7 +57,7 @@ struct InstDesc {
     uint64_t pstartR;        // offset (in uint32) of R partition starts (F+1 entries)
     uint64_t pstartS;        // offset of S partition starts (F+1 entries)
     uint32_t *heavy_keys;    // kHeavySlots keys (0 = empty)
-    uint32_t *heavy_rids;    // kHeavySlots R rids (kNoRid = none)
+    uint32_t *heavy_rids;    // kHeavySlots R rids, then kHeavyPresWords presence bits
     const uint32_t *heavy_count;   // number of heavy keys found (device)
     uint32_t table_log2;     // LDS hash table slots (log2) for this instance
     uint32_t alg;            // 0 hash (auto NL for tiny), 2 sort-merge, 3 nested loop

@@ -436,7 +436,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     SPJ_TRY(ctrl_.ensure(ctrl_words(n) * 4));
     if (nphj_words) SPJ_TRY(nphj_.ensure(nphj_words * 8));
     if (nheavy_inst) {
-        SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * 2 * kHeavySlots * 4));
+        SPJ_TRY(heavy_.ensure((size_t)nheavy_inst * kHeavyWords * 4));
         SPJ_TRY(cand_.ensure((size_t)nheavy_inst * cand_words() * 4));
     }
     for (size_t i = 0; i < n; i++) {
@@ -445,7 +445,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         d.Sp = sp_.as<uint2>() + sp_off[i];
         d.heavy_count = ctrl_.as<uint32_t>() + ctrl_at(n, kCtrlHeavy, i);
         if (heavy_idx[i] >= 0) {
-            d.heavy_keys = heavy_.as<uint32_t>() + (size_t)heavy_idx[i] * 2 * kHeavySlots;
+            d.heavy_keys = heavy_.as<uint32_t>() + (size_t)heavy_idx[i] * kHeavyWords;
             d.heavy_rids = d.heavy_keys + kHeavySlots;
         }
     }
@@ -497,7 +497,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         pa.ctrl = ctrl;
         pa.nctrl = ctrl_words(n);
         pa.heavy = heavy_.as<uint32_t>();
-        pa.nheavy = (uint64_t)nheavy_inst * 2 * kHeavySlots;
+        pa.nheavy = (uint64_t)nheavy_inst * kHeavyWords;
         pa.cand = cand_.as<uint32_t>();
         pa.ncand = (uint64_t)nheavy_inst * cand_words();
         pa.counts = counts_.as<uint32_t>();

@@ -35,6 +35,12 @@ JoinInstance::JoinInstance() : JoinInstance(16, 1, 16) {}
 
 JoinInstance::JoinInstance(int n) : JoinInstance(n, 1, n) {}
 
+JoinInstance::JoinInstance(int, int) : num_rank_allocated(0), num_rankgroup(0), num_rank_in_rankgroup(0), h_(nullptr)
+{
+    throw std::logic_error("JoinInstance(int num_rank_allocated, int rank_group_id) is deprecated; use "
+                           "JoinInstance(num_rank_allocated, num_rankgroup, num_rank_in_rankgroup)");
+}
+
 JoinInstance::JoinInstance(int n, int groups, int per_group)
     : num_rank_allocated(n), num_rankgroup(groups), num_rank_in_rankgroup(per_group), h_(nullptr)
 {

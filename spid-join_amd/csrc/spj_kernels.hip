@@ -164,14 +164,31 @@ __device__ __forceinline__ uint32_t heavy_keys_of(const InstDesc &I)
     return *reinterpret_cast<const_u32 *>(reinterpret_cast<uintptr_t>(I.heavy_count));
 }
 
-// Load the instance's heavy table (keys, R rids) into LDS; the caller
-// synchronises before using it.
-__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr)
+// Load the instance's heavy table into LDS: the keys, and with rids set
+// the R rids and the presence bitmap behind them (kHeavyWords - kHeavySlots
+// words at hr).  The caller synchronises before using it.
+__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr, bool rids)
 {
-    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        hk[i] = gld32(I.heavy_keys, i);
-        hr[i] = gld32(I.heavy_rids, i);
-    }
+    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) hk[i] = gld32(I.heavy_keys, i);
+    if (rids)
+        for (uint32_t i = threadIdx.x; i < kHeavySlots + kHeavyPresWords; i += blockDim.x)
+            hr[i] = gld32(I.heavy_rids, i);
+}
+
+// R side: slot s's key has an R tuple with this rid.  R keys are unique, so
+// each slot has one writer; the presence bit shares its word with 31 other
+// slots and is set atomically.
+__device__ __forceinline__ void heavy_record_rid(const InstDesc &I, uint32_t s, uint32_t rid)
+{
+    gst32(I.heavy_rids, s, rid);
+    atomicOr(I.heavy_rids + kHeavySlots + (s >> 5), 1u << (s & 31u));
+}
+
+// S side: whether slot s (>= 0) has an R tuple, from the presence bitmap
+// in LDS behind the rids.
+__device__ __forceinline__ bool heavy_present(const uint32_t *hr, int s)
+{
+    return (hr[kHeavySlots + ((uint32_t)s >> 5)] >> ((uint32_t)s & 31u)) & 1u;
 }
 
 // Slot of key (never 0: empty slots hold key 0) in the LDS heavy table, or
@@ -262,11 +279,10 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
     const InstDesc &I = insts[td.inst];
     const uint32_t F = 1u << I.bits, mask = F - 1u;
     uint32_t *hk = lds;                   // kHeavySlots (16-byte aligned)
-    uint32_t *hr = hk + kHeavySlots;      // kHeavySlots
-    uint32_t *hist = hr + kHeavySlots;    // F
+    uint32_t *hist = hk + kHeavySlots;    // F
     const bool heavy = I.heavy != 0 && heavy_keys_of(I) != 0;
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
-    if (heavy) load_heavy(I, hk, hr);
+    if (heavy) load_heavy(I, hk, nullptr, false);
     __syncthreads();
     const uint2 *in = td.rel ? I.S : I.R;
     const uint32_t mode = !heavy ? 0u : (td.rel == 0 ? 1u : 2u);
@@ -287,7 +303,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
             if (v[j].x == kEmptyKey) continue;
             if (mode == 2 && hs[j] >= 0) continue;   // heavy S tuple: no partition
             atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
-            if (mode == 1 && hs[j] >= 0) gst32(I.heavy_rids, hs[j], v[j].y);   // R keys are unique: one writer per slot
+            if (mode == 1 && hs[j] >= 0) heavy_record_rid(I, (uint32_t)hs[j], v[j].y);
         }
     }
     __syncthreads();
@@ -443,8 +459,8 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     const uint32_t FM = 1u << max_bits;
     // LDS layout
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
-    uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots
-    uint2 *stage = reinterpret_cast<uint2 *>(hr + kHeavySlots);     // ROUND tuples
+    uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots rids + presence bits
+    uint2 *stage = reinterpret_cast<uint2 *>(hr + kHeavySlots + kHeavyPresWords);   // ROUND tuples
     uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
     uint32_t *bs = lst;                                             // after staging: write base - start
     uint32_t *wtmp = lst + FM + 2;                                  // NT/64 + 1
@@ -500,7 +516,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         const uint32_t rn = min(ROUND, td.end - o);
         __syncthreads();   // the previous round's write-out and clearing are done
         if ((heavy || rec) && heavy_inst != td.inst) {
-            load_heavy(I, hk, hr);
+            load_heavy(I, hk, hr, true);
             heavy_inst = td.inst;
             __syncthreads();
         }
@@ -516,8 +532,12 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
             }
             uint32_t hrid[IPT];
+            bool hpres[IPT];
 #pragma unroll
-            for (int j = 0; j < IPT; j++) hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : kNoRid;
+            for (int j = 0; j < IPT; j++) {
+                hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : 0u;
+                hpres[j] = (hs[j] >= 0) && heavy_present(hr, hs[j]);
+            }
             uint64_t hm[IPT];
             uint32_t htot = 0;
 #pragma unroll
@@ -526,7 +546,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 dg[j] = DROP;
                 if (li < rn && v[j].x != kEmptyKey) {
                     if (hs[j] < 0) dg[j] = sfh(v[j].x) & mask;
-                    else if (hrid[j] != kNoRid) { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
+                    else if (hpres[j]) { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
                     // (a heavy key with no R tuple is dropped)
                 }
                 hm[j] = __ballot(dg[j] == F);
@@ -552,7 +572,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 }
 #pragma unroll
                 for (int j = 0; j < IPT; j++)
-                    if (hs[j] >= 0) gst32(I.heavy_rids, hs[j], v[j].y);   // R keys are unique: one writer per slot
+                    if (hs[j] >= 0) heavy_record_rid(I, (uint32_t)hs[j], v[j].y);
             }
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
@@ -906,10 +926,8 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
         }
         __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) {
-        gst32(I.heavy_keys, i, tab[i]);
-        gst32(I.heavy_rids, i, kNoRid);
-    }
+    // the rids and presence bits stay as k_prep zeroed them: no R tuple yet
+    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) gst32(I.heavy_keys, i, tab[i]);
     if (threadIdx.x == 0) nheavy[job.inst] = placed;
 }
 
@@ -1041,6 +1059,8 @@ constexpr uint32_t kNLMax = 32;             // nested loop when |R_p| <= 32 (aut
 constexpr uint32_t kBucketBits = 13;
 constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-bit ends: 16 KB)
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
+constexpr uint32_t kSubFill = 16384;        // mean sub-table fill when R_p needs several tables
+constexpr uint32_t kResplits = 2;           // sub-table overflows re-split (2Q) before sort-merge
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
 constexpr uint32_t kSortTail = 1024;        // sort-merge: a second, short run in the same pass
@@ -1225,9 +1245,17 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         const InstDesc &I = insts[w.inst];
         const uint32_t rb = w.rb, re = w.re;
         const uint32_t nr = re - rb;
+        // The algorithm by partition size: the nested loop for a tiny R
+        // partition, else the hash join (in sub-tables when R_p is larger
+        // than one LDS table); a partition the sub-tables cannot hold goes on
+        // to sort-merge, which takes any size.
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
         uint32_t *erri = err + w.inst;
+        // set (uniformly) once the item has taken the next item from the
+        // queue and its output reservation: the hash path does both at its
+        // first sub-table, the other paths at their start and end
+        bool advanced = false;
         if (threadIdx.x == 0) {
             sh.res = w.se - w.sb;
             sh.fill = 0;
@@ -1235,383 +1263,425 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         }
         __syncthreads();
 
-        if (alg == 3) {
-            // ---------------- nested loop: R chunks in LDS; every S tuple
-            // compares against every R key of the chunk (first match wins)
-            uint32_t *rk = lds, *rv = lds + kNLChunk;
-            for (uint32_t c0 = 0; c0 < nr; c0 += kNLChunk) {
-                const uint32_t cn = min(kNLChunk, nr - c0);
-                __syncthreads();
-                for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); rk[i] = r.x; rv[i] = r.y; }
-                __syncthreads();
-                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
-                    uint2 v[IPT];
-                    load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
-#pragma unroll
-                    for (int j = 0; j < (int)IPT; j++) {
-                        bool hit = false;
-                        uint32_t rr = 0;
-                        if (v[j].x != kEmptyKey) {
-                            for (uint32_t i = 0; i < cn; i++) {
-                                if (rk[i] == v[j].x) { rr = rv[i]; hit = true; break; }
-                            }
-                        }
-                        wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
-                    }
-                }
-            }
-        } else if (alg == 2) {
-            // ---------------- sort-merge (mpsm_join_sort_probe.c): the R
-            // partition is bitonic-sorted by key in LDS (runs of kSortR);
-            // each S tuple then takes the R entry at the lower bound of its
-            // key, found by binary search -- the entry the reference's merge
-            // of the two sorted runs (probe(), 246-272) pairs it with.  The
-            // merge walk needs S sorted; the search does not, so S is probed
-            // as it lies, IPT tuples per thread with their searches in
-            // lockstep.  Adjacent equal keys in a sorted run are duplicate R
-            // keys.
-            // A partition of up to kSortR + kSortTail tuples is one pass: a
-            // run of kSortR and a short second run (kSortR-tuple partitions
-            // overflow by a little about half the time; a second pass would
-            // read and search all of S again).
-            uint64_t *ra = reinterpret_cast<uint64_t *>(lds), *ra2 = ra + kSortPad;
-            for (uint32_t c0 = 0; c0 < nr;) {
-                const uint32_t cn = min(kSortR, nr - c0);
-                const uint32_t rest = nr - c0 - cn;
-                const uint32_t cn2 = rest <= kSortTail ? rest : 0u;
-                uint32_t np = 1, np2 = 1;
-                while (np < cn) np <<= 1;
-                while (np2 < cn2) np2 <<= 1;
-                __syncthreads();
-                for (uint32_t i = threadIdx.x; i < np; i += NT) {
-                    if (i < cn) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); ra[i] = ((uint64_t)r.x << 32) | r.y; }
-                    else ra[i] = ~0ull;
-                }
-                if (cn2) {
-                    for (uint32_t i = threadIdx.x; i < np2; i += NT) {
-                        if (i < cn2) { uint2 r = gld(I.Rp, rpos(rb + c0 + cn + i)); ra2[i] = ((uint64_t)r.x << 32) | r.y; }
-                        else ra2[i] = ~0ull;
-                    }
-                }
-                __syncthreads();
-                // each ends with a barrier
-                if (np == 16384) bitonic_sort_regs<16>(ra);
-                else if (np == 8192) bitonic_sort_regs<8>(ra);
-                else if (np == 4096) bitonic_sort_regs<4>(ra);
-                else bitonic_sort_lds(ra, np);
-                if (cn2) bitonic_sort_lds(ra2, np2);
-                // lower bound of key k in the sorted run x[0, n)
-                auto lower = [](const uint64_t *x, uint32_t n, uint32_t k) {
-                    uint32_t lo = 0, hi = n;
-                    while (lo < hi) {
-                        const uint32_t m = (lo + hi) >> 1;
-                        if ((uint32_t)(x[m] >> 32) < k) lo = m + 1; else hi = m;
-                    }
-                    return lo;
-                };
-                {
-                    bool dup = false;
-                    for (uint32_t i = threadIdx.x; i + 1 < cn; i += NT) {
-                        const uint32_t k = (uint32_t)(ra[i] >> 32);
-                        dup |= k != kEmptyKey && k == (uint32_t)(ra[i + 1] >> 32);
-                    }
-                    for (uint32_t i = threadIdx.x; i < cn2; i += NT) {   // the second run: itself and against the first
-                        const uint32_t k = (uint32_t)(ra2[i] >> 32);
-                        if (k == kEmptyKey) continue;
-                        dup |= i + 1 < cn2 && k == (uint32_t)(ra2[i + 1] >> 32);
-                        const uint32_t q = lower(ra, cn, k);
-                        dup |= q < cn && (uint32_t)(ra[q] >> 32) == k;
-                    }
-                    if (dup) atomicOr(erri, kErrDuplicateKey);
-                }
-                for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
-                    uint2 v[IPT];
-                    load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
-                    uint32_t lo[IPT], hi[IPT];
-#pragma unroll
-                    for (int j = 0; j < (int)IPT; j++) {
-                        lo[j] = 0;
-                        hi[j] = cn;
-                    }
-                    // floor(log2 cn) + 1 halvings close every [lo, hi)
-                    for (uint32_t step = cn; step > 0; step >>= 1) {
+        for (;;) {   // one pass; two when the hash path hands the item to sort-merge
+            if (alg == 3) {
+                // ---------------- nested loop: R chunks in LDS; every S tuple
+                // compares against every R key of the chunk (first match wins)
+                uint32_t *rk = lds, *rv = lds + kNLChunk;
+                for (uint32_t c0 = 0; c0 < nr; c0 += kNLChunk) {
+                    const uint32_t cn = min(kNLChunk, nr - c0);
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < cn; i += NT) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); rk[i] = r.x; rv[i] = r.y; }
+                    __syncthreads();
+                    for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                        uint2 v[IPT];
+                        load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
 #pragma unroll
                         for (int j = 0; j < (int)IPT; j++) {
-                            if (lo[j] < hi[j]) {
-                                const uint32_t m = (lo[j] + hi[j]) >> 1;
-                                if ((uint32_t)(ra[m] >> 32) < v[j].x) lo[j] = m + 1;
-                                else hi[j] = m;
+                            bool hit = false;
+                            uint32_t rr = 0;
+                            if (v[j].x != kEmptyKey) {
+                                for (uint32_t i = 0; i < cn; i++) {
+                                    if (rk[i] == v[j].x) { rr = rv[i]; hit = true; break; }
+                                }
                             }
+                            wave_emit(hit, rr, v[j].y, &sh, I.out, erri);
                         }
                     }
-                    uint32_t hitm = 0;
-                    uint2 pr[IPT];
-#pragma unroll
-                    for (int j = 0; j < (int)IPT; j++) {
-                        pr[j] = v[j];
-                        if (v[j].x != kEmptyKey && lo[j] < cn) {
-                            const uint64_t e = ra[lo[j]];
-                            if ((uint32_t)(e >> 32) == v[j].x) {
-                                hitm |= 1u << j;
-                                pr[j] = make_uint2((uint32_t)e, v[j].y);   // (R rid, S rid)
-                            }
+                }
+            } else if (alg == 2) {
+                // ---------------- sort-merge (mpsm_join_sort_probe.c): the R
+                // partition is bitonic-sorted by key in LDS (runs of kSortR);
+                // each S tuple then takes the R entry at the lower bound of its
+                // key, found by binary search -- the entry the reference's merge
+                // of the two sorted runs (probe(), 246-272) pairs it with.  The
+                // merge walk needs S sorted; the search does not, so S is probed
+                // as it lies, IPT tuples per thread with their searches in
+                // lockstep.  Adjacent equal keys in a sorted run are duplicate R
+                // keys.
+                // A partition of up to kSortR + kSortTail tuples is one pass: a
+                // run of kSortR and a short second run (kSortR-tuple partitions
+                // overflow by a little about half the time; a second pass would
+                // read and search all of S again).
+                uint64_t *ra = reinterpret_cast<uint64_t *>(lds), *ra2 = ra + kSortPad;
+                for (uint32_t c0 = 0; c0 < nr;) {
+                    const uint32_t cn = min(kSortR, nr - c0);
+                    const uint32_t rest = nr - c0 - cn;
+                    const uint32_t cn2 = rest <= kSortTail ? rest : 0u;
+                    uint32_t np = 1, np2 = 1;
+                    while (np < cn) np <<= 1;
+                    while (np2 < cn2) np2 <<= 1;
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < np; i += NT) {
+                        if (i < cn) { uint2 r = gld(I.Rp, rpos(rb + c0 + i)); ra[i] = ((uint64_t)r.x << 32) | r.y; }
+                        else ra[i] = ~0ull;
+                    }
+                    if (cn2) {
+                        for (uint32_t i = threadIdx.x; i < np2; i += NT) {
+                            if (i < cn2) { uint2 r = gld(I.Rp, rpos(rb + c0 + cn + i)); ra2[i] = ((uint64_t)r.x << 32) | r.y; }
+                            else ra2[i] = ~0ull;
                         }
                     }
-                    if (cn2) {   // keys not in the first run: the second
+                    __syncthreads();
+                    // each ends with a barrier
+                    if (np == 16384) bitonic_sort_regs<16>(ra);
+                    else if (np == 8192) bitonic_sort_regs<8>(ra);
+                    else if (np == 4096) bitonic_sort_regs<4>(ra);
+                    else bitonic_sort_lds(ra, np);
+                    if (cn2) bitonic_sort_lds(ra2, np2);
+                    // lower bound of key k in the sorted run x[0, n)
+                    auto lower = [](const uint64_t *x, uint32_t n, uint32_t k) {
+                        uint32_t lo = 0, hi = n;
+                        while (lo < hi) {
+                            const uint32_t m = (lo + hi) >> 1;
+                            if ((uint32_t)(x[m] >> 32) < k) lo = m + 1; else hi = m;
+                        }
+                        return lo;
+                    };
+                    {
+                        bool dup = false;
+                        for (uint32_t i = threadIdx.x; i + 1 < cn; i += NT) {
+                            const uint32_t k = (uint32_t)(ra[i] >> 32);
+                            dup |= k != kEmptyKey && k == (uint32_t)(ra[i + 1] >> 32);
+                        }
+                        for (uint32_t i = threadIdx.x; i < cn2; i += NT) {   // the second run: itself and against the first
+                            const uint32_t k = (uint32_t)(ra2[i] >> 32);
+                            if (k == kEmptyKey) continue;
+                            dup |= i + 1 < cn2 && k == (uint32_t)(ra2[i + 1] >> 32);
+                            const uint32_t q = lower(ra, cn, k);
+                            dup |= q < cn && (uint32_t)(ra[q] >> 32) == k;
+                        }
+                        // the tuples of the partition's earlier passes against
+                        // this pass's runs: a key repeated across passes is a
+                        // duplicate too (each pair of runs is compared once, when
+                        // the later one is sorted)
+                        for (uint32_t i = threadIdx.x; i < c0; i += NT) {
+                            const uint32_t k = gld(I.Rp, rpos(rb + i)).x;
+                            if (k == kEmptyKey) continue;
+                            const uint32_t q = lower(ra, cn, k);
+                            dup |= q < cn && (uint32_t)(ra[q] >> 32) == k;
+                            if (cn2) {
+                                const uint32_t q2 = lower(ra2, cn2, k);
+                                dup |= q2 < cn2 && (uint32_t)(ra2[q2] >> 32) == k;
+                            }
+                        }
+                        if (dup) atomicOr(erri, kErrDuplicateKey);
+                    }
+                    for (uint32_t s0 = w.sb; s0 < w.se; s0 += ROUND) {
+                        uint2 v[IPT];
+                        load_round<IPT>(v, I.Sp, spos, s0, w.se, threadIdx.x);
+                        uint32_t lo[IPT], hi[IPT];
 #pragma unroll
                         for (int j = 0; j < (int)IPT; j++) {
-                            if ((hitm >> j) & 1u || v[j].x == kEmptyKey) continue;
-                            const uint32_t q = lower(ra2, cn2, v[j].x);
-                            if (q < cn2 && (uint32_t)(ra2[q] >> 32) == v[j].x) {
-                                hitm |= 1u << j;
-                                pr[j] = make_uint2((uint32_t)ra2[q], v[j].y);
+                            lo[j] = 0;
+                            hi[j] = cn;
+                        }
+                        // floor(log2 cn) + 1 halvings close every [lo, hi)
+                        for (uint32_t step = cn; step > 0; step >>= 1) {
+#pragma unroll
+                            for (int j = 0; j < (int)IPT; j++) {
+                                if (lo[j] < hi[j]) {
+                                    const uint32_t m = (lo[j] + hi[j]) >> 1;
+                                    if ((uint32_t)(ra[m] >> 32) < v[j].x) lo[j] = m + 1;
+                                    else hi[j] = m;
+                                }
+                            }
+                        }
+                        uint32_t hitm = 0;
+                        uint2 pr[IPT];
+#pragma unroll
+                        for (int j = 0; j < (int)IPT; j++) {
+                            pr[j] = v[j];
+                            if (v[j].x != kEmptyKey && lo[j] < cn) {
+                                const uint64_t e = ra[lo[j]];
+                                if ((uint32_t)(e >> 32) == v[j].x) {
+                                    hitm |= 1u << j;
+                                    pr[j] = make_uint2((uint32_t)e, v[j].y);   // (R rid, S rid)
+                                }
+                            }
+                        }
+                        if (cn2) {   // keys not in the first run: the second
+#pragma unroll
+                            for (int j = 0; j < (int)IPT; j++) {
+                                if ((hitm >> j) & 1u || v[j].x == kEmptyKey) continue;
+                                const uint32_t q = lower(ra2, cn2, v[j].x);
+                                if (q < cn2 && (uint32_t)(ra2[q] >> 32) == v[j].x) {
+                                    hitm |= 1u << j;
+                                    pr[j] = make_uint2((uint32_t)ra2[q], v[j].y);
+                                }
+                            }
+                        }
+                        wave_emit_round<IPT>(hitm, pr, &sh, I.out, erri);
+                    }
+                    c0 += cn + cn2;
+                }
+            } else {
+                // ---------------- hash join
+                // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
+                // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
+                // bucket b is [bend[b], bend[b + 1]).
+                // During the count pass the 32-bit bucket counters live in the
+                // pair area, which the fill pass overwrites afterwards.
+                uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
+                uint32_t *bcnt = lds;                           // nb counters (count pass only)
+                uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
+                uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
+                // A partition larger than one table is split by high hash bits
+                // into Q sub-tables of kSubFill tuples on average (the slack up
+                // to kTableCap covers the spread of their sizes).  One that still
+                // overflows splits the partition again, into 2Q sub-tables, from
+                // the start; after kResplits such splits it goes to sort-merge.
+                uint32_t Q = nr <= kTableCap ? 1u : (nr + kSubFill - 1) / kSubFill;
+                uint32_t splits = 0;
+                bool to_smj = false;
+                const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
+                constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
+                for (uint32_t q = 0; q < Q;) {
+                    __syncthreads();
+                    {
+                        uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
+                        asm volatile("" : "+v"(i0));
+                        for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
+                    }
+                    __syncthreads();
+                    // R_p fits one table (Q == 1): hold it in registers (<= RPT
+                    // tuples per thread) and read it once; otherwise each
+                    // sub-table reads R_p twice, keeping its own keys
+                    uint2 r[RPT];
+#pragma unroll
+                    for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
+                    if (Q == 1) {
+#pragma unroll
+                        for (uint32_t u = 0; u < RPT; u++) {
+                            const uint32_t i = rb + u * NT + threadIdx.x;
+                            if (i < re) r[u] = gld(I.Rp, rpos(i));
+                        }
+#pragma unroll
+                        for (uint32_t u = 0; u < RPT; u++)
+                            if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bucket_of(r[u].x, lb)], 1u);
+                    } else {
+                        for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                            uint2 c[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const uint32_t i = i0 + u * NT + threadIdx.x;
+                                c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
                             }
                         }
                     }
-                    wave_emit_round<IPT>(hitm, pr, &sh, I.out, erri);
-                }
-                c0 += cn + cn2;
-            }
-        } else {
-            // ---------------- hash join
-            // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
-            // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
-            // bucket b is [bend[b], bend[b + 1]).
-            // During the count pass the 32-bit bucket counters live in the
-            // pair area, which the fill pass overwrites afterwards.
-            uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
-            uint32_t *bcnt = lds;                           // nb counters (count pass only)
-            uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
-            uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
-            const uint32_t Q = (nr + kTableCap - 1) / kTableCap;
-            const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
-            constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
-            uint32_t q = 0;   // Q >= 1
-            do {
-                __syncthreads();
-                {
-                    uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
-                    asm volatile("" : "+v"(i0));
-                    for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
-                }
-                __syncthreads();
-                // R_p fits one table (Q == 1): hold it in registers (<= RPT
-                // tuples per thread) and read it once; otherwise each
-                // sub-table reads R_p twice, keeping its own keys
-                uint2 r[RPT];
-#pragma unroll
-                for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
-                if (Q == 1) {
-#pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) {
-                        const uint32_t i = rb + u * NT + threadIdx.x;
-                        if (i < re) r[u] = gld(I.Rp, rpos(i));
+                    __syncthreads();
+                    const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                    if (ntab > kTableCap) {   // uniform: ntab is the same in every thread
+                        if (!advanced) {   // the queue and reservation atomics, as below
+                            uint32_t nx = 0, rbx = 0;
+                            if (threadIdx.x == 0) {
+                                nx = atomicAdd(&ctrl[1], 1u);
+                                rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                            }
+                            __builtin_amdgcn_s_waitcnt(0);
+                            if (threadIdx.x == 0) {
+                                sh.item = sh.item2;
+                                sh.item2 = nx;
+                                sh.base = rbx;
+                            }
+                            advanced = true;
+                        }
+                        // the earlier sub-tables' pairs are written again, from
+                        // the reservation's start (every barrier since their
+                        // probe has passed, so no wave still appends)
+                        if (threadIdx.x == 0) sh.fill = 0;
+                        if (splits == kResplits) {
+                            to_smj = true;
+                            break;
+                        }
+                        splits++;
+                        Q *= 2;
+                        q = 0;
+                        continue;
                     }
+                    // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
+                    // cursor of bucket b is bend[b + 1]
+                    {
+                        // the lane index is made opaque so that the addresses are
+                        // formed here: hoisted out of the item loop they spill,
+                        // and the reload waits (in order) for the previous item's
+                        // pair stores
+                        uint32_t i0 = threadIdx.x;
+                        asm volatile("" : "+v"(i0));
+                        for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                    }
+                    if (threadIdx.x == 0) bend[0] = 0;
+                    __syncthreads();
+                    // fill pass: a packed atomic advances one 16-bit cursor of a
+                    // word; it never carries, every cursor staying <= ntab
+                    if (Q == 1) {
 #pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++)
-                        if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bucket_of(r[u].x, lb)], 1u);
-                } else {
-                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                        uint2 c[4];
+                        for (uint32_t u = 0; u < RPT; u++) {
+                            if (r[u].x == kEmptyKey) continue;
+                            const uint32_t c = bucket_of(r[u].x, lb) + 1u, sh16 = (c & 1u) * 16u;
+                            const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                            tkv[pos] = r[u];
+                        }
+                    } else {
+                        for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                            uint2 c[4];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                            for (int u = 0; u < 4; u++) {
+                                const uint32_t i = i0 + u * NT + threadIdx.x;
+                                c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
+                                const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                                tkv[pos] = c[u];
+                            }
+                        }
+                    }
+                    if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
+                    __syncthreads();   // bend[b + 1] = end of bucket b
+                    // duplicate R keys (a2a_srj_build.c:265-271): each entry against
+                    // the later entries of its bucket, in phases (keys, bucket ends,
+                    // next entry) so that the LDS reads of all entries overlap;
+                    // buckets of more than two entries finish in a short loop
+                    {
+                        bool dup = false;
+                        uint32_t kk[RPT], ee[RPT];
+#pragma unroll
+                        for (uint32_t u = 0; u < RPT; u++) {
+                            const uint32_t i = u * NT + threadIdx.x;
+                            kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
                         }
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                            atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
+                        for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
+                        // the next kScan entries are read unconditionally (the
+                        // table has kScan slack slots past its last entry), so
+                        // that the reads overlap; only buckets longer than that
+                        // finish in a loop
+#pragma unroll
+                        for (uint32_t u = 0; u < RPT; u++) {
+                            const uint32_t i = u * NT + threadIdx.x;
+                            if (i >= ntab) continue;
+#pragma unroll
+                            for (uint32_t d = 1; d <= kScan; d++) {
+                                const uint32_t kd = tkv[i + d].x;   // i + d < ntab + kScan: inside the table
+                                dup |= (i + d < ee[u]) & (kd == kk[u]);
+                            }
                         }
+#pragma unroll
+                        for (uint32_t u = 0; u < RPT; u++) {
+                            const uint32_t i = u * NT + threadIdx.x;
+                            for (uint32_t j = i + kScan + 1; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
+                        }
+                        if (dup) atomicOr(erri, kErrDuplicateKey);
                     }
-                }
-                __syncthreads();
-                const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
-                if (ntab > kTableCap) {
-                    if (threadIdx.x == 0) atomicOr(erri, kErrTableOverflow);
-                    uint32_t nx = 0, rbx = 0;   // the queue and reservation atomics, as below
-                    if (q == 0 && threadIdx.x == 0) {
+                    // probe: the S range is cut into chunks of a wave's 64 x IPT
+                    // tuples; wave i starts on chunk i and then takes chunks from
+                    // a counter, so that waves that finish early take over the
+                    // rest instead of waiting at the item's end for slow ones.
+                    // The next chunk's loads are in flight while one is probed.
+                    constexpr uint32_t CH = kWave * IPT;
+                    const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
+                    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+                    const uint32_t ln = lane_id();
+                    uint2 nv[IPT];
+                    {
+                        // the lane index is made opaque so that the first chunk's
+                        // positions are mapped here, not hoisted to the item's
+                        // start (where they would spill, and their reload wait,
+                        // in order, behind every store in flight)
+                        uint32_t lane = ln;
+                        asm volatile("" : "+v"(lane));
+                        load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
+                    }
+                    uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
+                    if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
+                    uint32_t nx = 0, rbx = 0, pre = 0;
+                    // the next item's words: wave 0, lane i loads word i
+                    const uint32_t it2 = !advanced && threadIdx.x < kItemWords ? sh.item2 : total_items;
+                    if (it2 < total_items) {
+                        // the lane offset is made opaque so that the address is
+                        // formed here, not hoisted out of the item loop (a
+                        // 64-bit per-lane pointer there would spill)
+                        uint32_t lane = threadIdx.x;
+                        asm volatile("" : "+v"(lane));
+                        pre = gld32(reinterpret_cast<const uint32_t *>(items + it2), lane);
+                    }
+                    if (!advanced && threadIdx.x == 0) {
                         nx = atomicAdd(&ctrl[1], 1u);
                         rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
                     }
-                    __builtin_amdgcn_s_waitcnt(0);
-                    if (q == 0 && threadIdx.x == 0) {
-                        sh.item = sh.item2;
-                        sh.item2 = nx;
-                        sh.base = rbx;
-                    }
-                    continue;   // uniform: ntab is the same in every thread
-                }
-                // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
-                // cursor of bucket b is bend[b + 1]
-                {
-                    // the lane index is made opaque so that the addresses are
-                    // formed here: hoisted out of the item loop they spill,
-                    // and the reload waits (in order) for the previous item's
-                    // pair stores
-                    uint32_t i0 = threadIdx.x;
-                    asm volatile("" : "+v"(i0));
-                    for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
-                }
-                if (threadIdx.x == 0) bend[0] = 0;
-                __syncthreads();
-                // fill pass: a packed atomic advances one 16-bit cursor of a
-                // word; it never carries, every cursor staying <= ntab
-                if (Q == 1) {
-#pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) {
-                        if (r[u].x == kEmptyKey) continue;
-                        const uint32_t c = bucket_of(r[u].x, lb) + 1u, sh16 = (c & 1u) * 16u;
-                        const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
-                        tkv[pos] = r[u];
-                    }
-                } else {
-                    for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                        uint2 c[4];
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const uint32_t i = i0 + u * NT + threadIdx.x;
-                            c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                    __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
+                    if (!advanced) {
+                        if (it2 < total_items) nw[threadIdx.x] = pre;
+                        if (threadIdx.x == 0) {
+                            if (it2 < total_items) nw_item = it2;
+                            sh.item = sh.item2;   // read at the next item's start
+                            sh.item2 = nx;
+                            sh.base = rbx;        // read by the pair emission
                         }
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                            const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
-                            const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
-                            tkv[pos] = c[u];
-                        }
+                        __syncthreads();
+                        advanced = true;
                     }
-                }
-                if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
-                __syncthreads();   // bend[b + 1] = end of bucket b
-                // duplicate R keys (a2a_srj_build.c:265-271): each entry against
-                // the later entries of its bucket, in phases (keys, bucket ends,
-                // next entry) so that the LDS reads of all entries overlap;
-                // buckets of more than two entries finish in a short loop
-                {
-                    bool dup = false;
-                    uint32_t kk[RPT], ee[RPT];
+                    uint32_t cn = wv < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;   // this wave's next chunk
+                    for (uint32_t c = wv; c < nch;) {
+                        uint2 v[IPT];
 #pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) {
-                        const uint32_t i = u * NT + threadIdx.x;
-                        kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
-                    }
+                        for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
+                        if (cn < nch) load_chunk<IPT>(nv, I.Sp, spos, w.sb + cn * CH, w.se, ln);
+                        cg = 0;
+                        if (ln == 0 && cn < nch) cg = atomicAdd(&sh.chunk, 1u);   // the chunk after that
+                        uint32_t hitm = 0;
 #pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
-                    // the next kScan entries are read unconditionally (the
-                    // table has kScan slack slots past its last entry), so
-                    // that the reads overlap; only buckets longer than that
-                    // finish in a loop
+                        for (int j = 0; j < (int)IPT; j++) {
+                            // branch-free: key 0 and padding tuples simply find
+                            // no entry (the table holds no key 0), nor does a key
+                            // of another sub-table of a split partition; table
+                            // keys are unique (duplicates are an error), so at
+                            // most one entry matches
+                            const uint32_t k = v[j].x;
+                            const uint32_t bk = bucket_of(k, lb);
+                            const uint32_t st = bend[bk], n = bend[bk + 1] - st;
+                            bool hit = false;
+                            uint32_t rr = 0;
 #pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) {
-                        const uint32_t i = u * NT + threadIdx.x;
-                        if (i >= ntab) continue;
-#pragma unroll
-                        for (uint32_t d = 1; d <= kScan; d++) {
-                            const uint32_t kd = tkv[i + d].x;   // i + d < ntab + kScan: inside the table
-                            dup |= (i + d < ee[u]) & (kd == kk[u]);
-                        }
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) {
-                        const uint32_t i = u * NT + threadIdx.x;
-                        for (uint32_t j = i + kScan + 1; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
-                    }
-                    if (dup) atomicOr(erri, kErrDuplicateKey);
-                }
-                // probe: the S range is cut into chunks of a wave's 64 x IPT
-                // tuples; wave i starts on chunk i and then takes chunks from
-                // a counter, so that waves that finish early take over the
-                // rest instead of waiting at the item's end for slow ones.
-                // The next chunk's loads are in flight while one is probed.
-                constexpr uint32_t CH = kWave * IPT;
-                const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
-                const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
-                const uint32_t ln = lane_id();
-                uint2 nv[IPT];
-                {
-                    // the lane index is made opaque so that the first chunk's
-                    // positions are mapped here, not hoisted to the item's
-                    // start (where they would spill, and their reload wait,
-                    // in order, behind every store in flight)
-                    uint32_t lane = ln;
-                    asm volatile("" : "+v"(lane));
-                    load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
-                }
-                uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
-                if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
-                uint32_t nx = 0, rbx = 0, pre = 0;
-                // the next item's words: wave 0, lane i loads word i
-                const uint32_t it2 = q == 0 && threadIdx.x < kItemWords ? sh.item2 : total_items;
-                if (it2 < total_items) {
-                    // the lane offset is made opaque so that the address is
-                    // formed here, not hoisted out of the item loop (a
-                    // 64-bit per-lane pointer there would spill)
-                    uint32_t lane = threadIdx.x;
-                    asm volatile("" : "+v"(lane));
-                    pre = gld32(reinterpret_cast<const uint32_t *>(items + it2), lane);
-                }
-                if (q == 0 && threadIdx.x == 0) {
-                    nx = atomicAdd(&ctrl[1], 1u);
-                    rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
-                }
-                __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
-                if (q == 0) {
-                    if (it2 < total_items) nw[threadIdx.x] = pre;
-                    if (threadIdx.x == 0) {
-                        if (it2 < total_items) nw_item = it2;
-                        sh.item = sh.item2;   // read at the next item's start
-                        sh.item2 = nx;
-                        sh.base = rbx;        // read by the pair emission
-                    }
-                    __syncthreads();
-                }
-                uint32_t cn = wv < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;   // this wave's next chunk
-                for (uint32_t c = wv; c < nch;) {
-                    uint2 v[IPT];
-#pragma unroll
-                    for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
-                    if (cn < nch) load_chunk<IPT>(nv, I.Sp, spos, w.sb + cn * CH, w.se, ln);
-                    cg = 0;
-                    if (ln == 0 && cn < nch) cg = atomicAdd(&sh.chunk, 1u);   // the chunk after that
-                    uint32_t hitm = 0;
-#pragma unroll
-                    for (int j = 0; j < (int)IPT; j++) {
-                        // branch-free: key 0 and padding tuples simply find
-                        // no entry (the table holds no key 0), nor does a key
-                        // of another sub-table of a split partition; table
-                        // keys are unique (duplicates are an error), so at
-                        // most one entry matches
-                        const uint32_t k = v[j].x;
-                        const uint32_t bk = bucket_of(k, lb);
-                        const uint32_t st = bend[bk], n = bend[bk + 1] - st;
-                        bool hit = false;
-                        uint32_t rr = 0;
-#pragma unroll
-                        for (uint32_t u = 0; u < kScan; u++) {
-                            const uint2 e = tkv[st + u];   // the table has kScan padding slots
-                            const bool m = (u < n) && (e.x == k);
-                            rr = m ? e.y : rr;
-                            hit = hit || m;
-                        }
-                        if (n > kScan && !hit) {   // a long bucket (rare)
-                            for (uint32_t i = st + kScan; i < st + n; i++) {
-                                const uint2 e = tkv[i];
-                                if (e.x == k) { rr = e.y; hit = true; break; }
+                            for (uint32_t u = 0; u < kScan; u++) {
+                                const uint2 e = tkv[st + u];   // the table has kScan padding slots
+                                const bool m = (u < n) && (e.x == k);
+                                rr = m ? e.y : rr;
+                                hit = hit || m;
                             }
+                            if (n > kScan && !hit) {   // a long bucket (rare)
+                                for (uint32_t i = st + kScan; i < st + n; i++) {
+                                    const uint2 e = tkv[i];
+                                    if (e.x == k) { rr = e.y; hit = true; break; }
+                                }
+                            }
+                            v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
+                            hitm |= (hit ? 1u : 0u) << j;
                         }
-                        v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
-                        hitm |= (hit ? 1u : 0u) << j;
+                        // wait here for the next chunk's loads (issued a probe
+                        // ago), not at the next chunk's start, where the wait
+                        // would also cover the stores below
+                        __builtin_amdgcn_s_waitcnt(0);
+                        c = cn;
+                        cn = cn < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;
+                        wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
                     }
-                    // wait here for the next chunk's loads (issued a probe
-                    // ago), not at the next chunk's start, where the wait
-                    // would also cover the stores below
-                    __builtin_amdgcn_s_waitcnt(0);
-                    c = cn;
-                    cn = cn < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;
-                    wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
+                    q++;
                 }
-            } while (++q < Q);
+                if (to_smj) {   // sh.fill was reset above
+                    alg = 2;
+                    continue;
+                }
+            }
+            break;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1621,7 +1691,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 const uint32_t h = atomicAdd(&ctrl[2], 1u);
                 if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
             }
-            if (alg != 0) {   // the hash path advanced the queue at its probe's start
+            if (!advanced) {   // (the hash path took the next item at its first sub-table)
                 sh.item = sh.item2;
                 sh.item2 = atomicAdd(&ctrl[1], 1u);
             }
@@ -1788,13 +1858,13 @@ static hipError_t set_lds(const void *fn, size_t bytes)
 
 size_t lds_hist(uint32_t max_bits)
 {
-    return ((size_t(1) << max_bits) + 2 * kHeavySlots) * 4;
+    return ((size_t(1) << max_bits) + kHeavySlots) * 4;
 }
 
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
-    return 2 * kHeavySlots * 4 + R * 8 + (F + 2) * 4 +
+    return (size_t)kHeavyWords * 4 + R * 8 + (F + 2) * 4 +
            (kScatterNT / kWave + 1) * 4 + 16;
 }
 
@@ -1963,5 +2033,6 @@ hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *seg
     hipLaunchKernelGGL(k_gather_segments, dim3(grid), dim3(256), 0, st, src, dst, segs, nsegs);
     return hipGetLastError();
 }
+
 
 }  // namespace spj

@@ -23,7 +23,8 @@ ERRORS = {0: "OK", -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EDUPKEY", -5: "E
 # Every function the C header declares (checked by the CPU tests).
 EXPORTS = [
     "spj_config_default", "spj_last_error", "spj_init", "spj_device_info", "spj_dev_alloc", "spj_dev_free",
-    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_pairs_checksum_device", "spj_dev_sync", "spj_join_device", "spj_join_host",
+    "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_pairs_checksum_device", "spj_dev_sync", "spj_join_device",
+    "spj_partition_device", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_set_query_plan", "spj_plan_check",
     "spj_timeline", "spj_free_result", "spj_destroy",
@@ -92,6 +93,7 @@ def lib():
         "spj_pairs_checksum_device": (ctypes.c_int, [P, U64, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
         "spj_join_device": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config),
                                            ctypes.POINTER(PhaseTimes)]),
+        "spj_partition_device": (ctypes.c_int, [P, U64, ctypes.c_int, P, P]),
         "spj_join_host": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config)]),
         "spj_join_batch_device": (ctypes.c_int, [ctypes.c_int, P, P, P, P, P, P, ctypes.POINTER(Config),
                                                  ctypes.POINTER(PhaseTimes)]),
@@ -245,6 +247,24 @@ def join(R, S, cfg=None, timed=False):
         dR.free()
         dS.free()
         dO.free()
+
+
+def partition(rel, radix_bits):
+    """The global partition pass on the GPU (spj_partition_device): returns
+    (partitioned tuples, partition starts of 2^radix_bits + 1 entries)."""
+    n = len(rel)
+    F = 1 << radix_bits
+    d_in = DeviceArray.from_numpy(rel)
+    d_out = DeviceArray(max(n, 1) * 8)
+    d_pst = DeviceArray((F + 1) * 4)
+    try:
+        check(lib().spj_partition_device(d_in.ptr if n else None, n, radix_bits, d_out.ptr, d_pst.ptr))
+        pst = d_pst.to_numpy(np.uint32, F + 1)
+        return d_out.to_numpy(TUPLE, int(pst[-1])), pst
+    finally:
+        d_in.free()
+        d_out.free()
+        d_pst.free()
 
 
 def join_host(R, S, cfg=None):

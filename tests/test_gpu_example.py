@@ -19,7 +19,17 @@ def test_reference_program_flow(args):
     r = subprocess.run([EXE] + args, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
-    assert "bad 0" in r.stdout
+    assert "bad 0 repeated 0 missing 0" in r.stdout
     # RecordLogs: one line per node of the plan the design selects
     assert "-COMPOUND_FUNC_" in r.stdout and "-HOST_FUNC_RECV_DATA_OPT-" in r.stdout
     assert ("-COMPOUND_FUNC_BG_BROADCAST-" in r.stdout) == ("-B" in args)
+
+
+def test_deprecated_constructor_throws():
+    """JoinInstance(int, int) is [[deprecated]] in the reference and exits
+    there (query.cc:733-736); here it throws std::logic_error."""
+    if not os.path.exists(EXE):
+        pytest.fail("join_example not built (run __graft_entry__.build())")
+    r = subprocess.run([EXE, "-d"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert "deprecated constructor refused" in r.stdout

@@ -520,3 +520,108 @@ def test_full_size_checksum(gpu, orc, nR, nS, zipf, alg):
         dS.free()
         dO.free()
     assert got == want
+
+
+def hottest_keys(S, k):
+    keys, counts = np.unique(S["key"], return_counts=True)
+    return keys[np.argsort(-counts)[:k]]
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_heavy_keys_with_full_range_r_rids(gpu, orc, layout):
+    """R rids are full-range uint32 values (kv_pair_t.rvalue): the hottest
+    keys of a Zipf 1.5 S, which the heavy path joins inside the scatter, have
+    R rids 0xFFFFFFFF and other extreme values; every other rid is random.
+    Counted and cell layouts, heavy handling on."""
+    sj = gpu
+    rng = np.random.default_rng(2024)
+    nR = 1 << 20
+    R = sj.gen_r(nR, 131)
+    S = sj.gen_s(1 << 22, R, 1.5, 132)
+    R["rid"] = rng.integers(0, 1 << 32, nR, dtype=np.uint64).astype(np.uint32)
+    hot = hottest_keys(S, 8)
+    pos = {int(k): i for i, k in enumerate(R["key"])}
+    for j, k in enumerate(hot):
+        R["rid"][pos[int(k)]] = [0xFFFFFFFF, 0, 0xFFFFFFFE, 0x80000000][j % 4]
+    print(f"case heavy full-range rids layout={layout}", flush=True)
+    pairs, t = sj.join(R, S, sj.config(algorithm=0, heavy=1, part_layout=layout), timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    assert t["heavy_keys"] >= 8
+    assert t["s_partitioned"] < len(S) // 2   # the hot keys went through the heavy path
+    # the same with a heavy key that has no R tuple (its S tuples are dropped)
+    R2 = R[R["key"] != hot[1]]
+    pairs, _ = sj.join(R2, S, sj.config(algorithm=0, heavy=1, part_layout=layout))
+    check_equal(sj, orc, R2, S, pairs)
+
+
+@pytest.mark.parametrize("bits", [0, 1, 2])
+def test_sort_merge_duplicate_across_runs(gpu, orc, bits):
+    """Sort-merge partitions of several sorted runs (2^17 R tuples over 1, 2
+    or 4 partitions: 2 to 8 passes of 16K): a key repeated in two different
+    runs of one partition is a duplicate (SPJ_EDUPKEY), as is one repeated
+    inside a run; without it the result matches the oracle."""
+    sj = gpu
+    R = sj.gen_r(1 << 17, 140 + bits)
+    S = sj.gen_s(1 << 18, R, 0.3, 150 + bits)
+    print(f"case smj runs bits={bits}", flush=True)
+    pairs, _ = sj.join(R, S, sj.config(algorithm=2, radix_bits=bits))
+    check_equal(sj, orc, R, S, pairs)
+    # partition of each key (the oracle's SuperFastHash, low bits)
+    part = orc.hash_batch("glb_partition_hash", R["key"]) & np.uint32((1 << bits) - 1)
+    p0 = np.flatnonzero(part == 0)
+    assert len(p0) > 16384 + 1024   # more than one pass
+    # Input tiles of 8192 tuples are dealt to 8 scatter groups in turn, and a
+    # partition holds group 0's tuples first and group 7's last: a key early
+    # in the input and one late in it sit in the partition's first and last
+    # runs.  The second pair may share a run.
+    for a, b in [(p0[10], p0[-10]), (p0[5], p0[len(p0) // 2])]:
+        R2 = R.copy()
+        R2["key"][b] = R2["key"][a]
+        with pytest.raises(sj.SpjError) as e:
+            sj.join(R2, S, sj.config(algorithm=2, radix_bits=bits))
+        assert e.value.code == -4, (a, b)
+
+
+def sub_table(h, bits, Q):
+    """The sub-table of an R tuple with SuperFastHash h in a partition split
+    Q ways (high hash bits, multiplied by 0x9E3779B1; the top of the 64-bit
+    product with Q).  Used only to build inputs that crowd one sub-table."""
+    x = ((h.astype(np.uint64) >> np.uint64(bits)) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    return (x * np.uint64(Q)) >> np.uint64(32)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_oversized_partition_resplit_and_sort_merge_fallback(gpu, orc, layout):
+    """R partitions larger than one LDS table, with keys chosen so that a
+    sub-table overflows: once (the partition is split again, into twice the
+    sub-tables) and for good (every key in the same eighth of the hash
+    range: the partition goes to sort-merge).  Same multiset as the oracle,
+    and a duplicate key in the sort-merge fallback is still reported."""
+    sj = gpu
+    cand = np.arange(1, 1 << 22, dtype=np.uint32)
+    h = orc.hash_batch("glb_partition_hash", cand)
+    # radix_bits 0: one partition; Q starts at ceil(n / 16384)
+    # (a) 20000 keys, all in sub-table 0 of 2 and of 4, spread over 8
+    #     sub-tables below: re-split twice, then fits
+    s8 = sub_table(h, 0, 8)
+    a = np.concatenate([cand[s8 == 0][:10000], cand[s8 == 1][:10000]])
+    # (b) 20000 keys in sub-table 0 of 8 (and so of 2 and 4): sort-merge
+    b = cand[s8 == 0][:20000]
+    rng = np.random.default_rng(7)
+    for name, keys in (("resplit", a), ("fallback", b)):
+        keys = rng.permutation(keys)
+        R = np.zeros(len(keys), dtype=sj.TUPLE)
+        R["key"] = keys
+        R["rid"] = np.arange(len(keys), dtype=np.uint32)
+        S = np.zeros(len(keys) * 3, dtype=sj.TUPLE)
+        S["key"] = keys[rng.integers(0, len(keys), len(S))]
+        S["rid"] = np.arange(len(S), dtype=np.uint32)
+        print(f"case oversized {name} layout={layout}", flush=True)
+        cfg = sj.config(algorithm=0, radix_bits=0, part_layout=layout, heavy=0)
+        pairs, _ = sj.join(R, S, cfg)
+        check_equal(sj, orc, R, S, pairs)
+        R2 = R.copy()
+        R2["key"][3] = R2["key"][len(R2) - 5]
+        with pytest.raises(sj.SpjError) as e:
+            sj.join(R2, S, cfg)
+        assert e.value.code == -4
