@@ -108,6 +108,20 @@ int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum
 int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
                     uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg, spj_phase_times_t *times);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spj_join_device's no-partition join (SPJ_ALG_NPHJ) with
+ * its table, descriptors and control words in a static device array of
+ * 16 MiB + 512 bytes that each device holds from library load until the
+ * process ends.  Nothing is allocated.  |R| <= 2^20 (SPJ_NOALLOC_MAX_R):
+ * larger R is refused with SPJ_EINVAL.  Every call on a device shares that
+ * device's array; calls in one process take turns (a lock), and the array
+ * is set up again by each call.  out must hold out_cap >= nS pairs; runs on
+ * the null stream and returns when the join is done. */
+#define SPJ_NOALLOC_MAX_R (1u << 20)
+int spj_join_device_noalloc(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                            uint64_t out_cap, uint64_t *out_count);
+
 /* The global partition pass alone, on one device-resident relation
  * (COMPOUND_FUNC_GLB_PARTITION, queryCompoundFunc.cc:814-872: GLB_PARTITION_COUNT,
  * the page histogram, GLB_PARTITION_PACKET).  Tuple t goes to partition

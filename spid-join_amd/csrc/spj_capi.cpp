@@ -163,6 +163,9 @@ int spj_init(int device)
     e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     (void)device_info();
+    void *scratch = nullptr;   // the no-allocation join's array on this device
+    e = scratch_on_device(&scratch);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
     return SPJ_OK;
 }
 
@@ -270,6 +273,29 @@ int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uin
     fill_times(t, times);
     *out_count = counts[0];
     return err_from_flags(errs[0], 0);
+}
+
+static_assert(SPJ_NOALLOC_MAX_R == kNoAllocMaxR, "the header states the kernels' limit");
+
+int spj_join_device_noalloc(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
+                            uint64_t out_cap, uint64_t *out_count)
+{
+    if ((!R && nR) || (!S && nS) || (!out && nS) || !out_count) return fail(SPJ_EINVAL, "null argument");
+    if (out_cap < nS) return fail(SPJ_EINVAL, "out_cap %llu < |S| %llu", (unsigned long long)out_cap, (unsigned long long)nS);
+    if (nR > kNoAllocMaxR)
+        return fail(SPJ_EINVAL, "spj_join_device_noalloc: |R| = %llu exceeds its table's limit of %u tuples",
+                    (unsigned long long)nR, kNoAllocMaxR);
+    int rc = validate_sizes(nR, nS);
+    if (rc) return rc;
+    // calls in this process take turns on the device's array
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    uint32_t ctrl[3] = {0, 0, 0};
+    hipError_t e = run_nphj_noalloc(reinterpret_cast<const uint2 *>(R), (uint32_t)nR, reinterpret_cast<const uint2 *>(S),
+                                    (uint32_t)nS, reinterpret_cast<uint2 *>(out), ctrl, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "no-allocation join");
+    *out_count = ctrl[1];
+    return err_from_flags(ctrl[2], 0);
 }
 
 int spj_partition_device(const spj_tuple_t *rel, uint64_t n, int radix_bits, spj_tuple_t *out, uint32_t *pstart)

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "spj_common.hpp"
 #include "spj_kernels.hpp"
@@ -2034,5 +2035,93 @@ hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *seg
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The no-allocation join: the no-partition hash join of one instance with
+// its descriptors, control words and table in a static device array, for
+// callers that cannot allocate.  Layout of `scratch` (8-byte words):
+//   [0, kNaHead)       InstDesc, NphjJob, then the control words out_lo,
+//                      out_fill and err (3 x uint32)
+//   [kNaHead, ...)     the open-addressing table, 2^log2cap slots
+// Host code reaches the array only through the address hipGetSymbolAddress
+// returns on each device (scratch_on_device); its name in host code is not
+// a device address.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNaHead = 64;   // 512 bytes of descriptors and control words
+constexpr uint32_t kNaMaxLog2Cap = 21;   // >= 2 |R| slots for |R| <= kNoAllocMaxR
+static_assert((uint64_t(1) << kNaMaxLog2Cap) >= 2 * kNoAllocMaxR, "the table holds 2 |R| slots");
+static_assert(sizeof(InstDesc) + sizeof(NphjJob) + 3 * 4 <= kNaHead * 8, "descriptors fit the head");
+constexpr uint64_t kNaWords = kNaHead + (uint64_t(1) << kNaMaxLog2Cap);
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+
+// Write the descriptors (kernel arguments) into the head and zero the
+// control words: no host-to-device copy is needed.
+__global__ void k_noalloc_setup(InstDesc d, NphjJob j, unsigned long long *head)
+{
+    if (threadIdx.x != 0) return;
+    *reinterpret_cast<InstDesc *>(head) = d;
+    *reinterpret_cast<NphjJob *>(reinterpret_cast<char *>(head) + sizeof(InstDesc)) = j;
+    uint32_t *ctrl = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(head) + sizeof(InstDesc) + sizeof(NphjJob));
+    ctrl[0] = ctrl[1] = ctrl[2] = 0u;
+}
+
+// The scratch array's device address on the current device, looked up once
+// per device (each device holds its own copy of the array).
+static std::mutex g_na_mu;
+static void *g_na_ptr[kNoAllocMaxDevices] = {};
+
+hipError_t scratch_on_device(void **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= (int)kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_na_ptr[dev] = p;
+    }
+    *ptr = g_na_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t run_nphj_noalloc(const uint2 *R, uint32_t nR, const uint2 *S, uint32_t nS, uint2 *out, uint32_t ctrl_out[3],
+                            hipStream_t st)
+{
+    if (nR > kNoAllocMaxR) return hipErrorInvalidValue;
+    void *base = nullptr;
+    hipError_t e = scratch_on_device(&base);
+    if (e != hipSuccess) return e;
+    auto *head = reinterpret_cast<unsigned long long *>(base);
+    InstDesc d{};
+    d.R = R;
+    d.S = S;
+    d.out = out;
+    d.nR = nR;
+    d.nS = nS;
+    d.alg = 1;
+    NphjJob j{};
+    j.inst = 0;
+    j.log2cap = 10;
+    while ((1ull << j.log2cap) < 2ull * nR) j.log2cap++;
+    j.table = 0;
+    j.first_build = 0;
+    j.first_probe = 0;
+    const InstDesc *d_inst = reinterpret_cast<const InstDesc *>(head);
+    const NphjJob *d_job = reinterpret_cast<const NphjJob *>(reinterpret_cast<char *>(head) + sizeof(InstDesc));
+    uint32_t *ctrl = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(head) + sizeof(InstDesc) + sizeof(NphjJob));
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(64), 0, st, d, j, head);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t bb = (nR + kNphjBuildTuples - 1) / kNphjBuildTuples;
+    const uint32_t pb = (uint32_t)(((uint64_t)nS + kNphjProbeTuples - 1) / kNphjProbeTuples);
+    e = launch_nphj(d_inst, d_job, 1, reinterpret_cast<uint64_t *>(head + kNaHead), uint64_t(1) << j.log2cap, bb, pb,
+                    ctrl, ctrl + 1, ctrl + 2, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(ctrl_out, ctrl, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
 
 }  // namespace spj

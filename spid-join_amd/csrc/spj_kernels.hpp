@@ -54,6 +54,18 @@ hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *c
 hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njobs, uint64_t *tables,
                        uint64_t table_words, uint32_t build_blocks, uint32_t probe_blocks, uint32_t *out_lo,
                        uint32_t *out_fill, uint32_t *err, hipStream_t st);
+// The no-allocation join (no partition pass) of one instance of at most
+// kNoAllocMaxR R tuples: descriptors, control words and table live in a
+// static device array (one per device, held until the process ends), so
+// nothing is allocated.  ctrl_out receives (pairs reserved, pairs written,
+// error flags).  The call waits for the stream.  Calls share the device's
+// array: they must not overlap on one device.
+constexpr uint32_t kNoAllocMaxR = 1u << 20;
+constexpr uint32_t kNoAllocMaxDevices = 64;
+hipError_t run_nphj_noalloc(const uint2 *R, uint32_t nR, const uint2 *S, uint32_t nS, uint2 *out, uint32_t ctrl_out[3],
+                            hipStream_t st);
+// Look up (once per device) the static array's address on the current device.
+hipError_t scratch_on_device(void **ptr);
 hipError_t launch_pairs_checksum(const uint2 *p, uint64_t n, unsigned long long *acc, hipStream_t st);
 hipError_t launch_gather_segments(const uint2 *src, uint2 *dst, const uint4 *segs, uint32_t nsegs, hipStream_t st);
 
