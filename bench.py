@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: the radix-partitioned equi-join on one MI355X.
 
-Default workload (one step = one full join on HBM-resident inputs):
+Default line (one step = one full join on HBM-resident inputs):
   |R| = 2^26 unique uint32 keys (permutation of 1..2^26, rid = row),
-  |S| = 2^27 keys drawn Zipf 0.5 over R's keys, hash join.
-Prints one JSON line: joined tuples/s = (|R| + |S|) / step time, plus the
-HBM roofline of the dominant kernel and a CPU baseline (the CPU restatement
-of the reference join on a bounded sample, on the host cores).
+  |S| = 2^28 keys drawn uniformly (Zipf 0) from R's keys, hash join.
+It prints one JSON line: joined tuples/s = (|R| + |S|) / step time, the HBM
+roofline of the dominant kernel (HIP events around each phase of every timed
+join), and a CPU baseline (the CPU restatement of the reference join on the
+reference's CPU-runnable workload, 2^20 x 2^21, on the host cores).
+`extra` holds the same join at Zipf 1.0 and 1.5 (and the Zipf 1.5 / Zipf 0
+time ratio), the 2^26 x 2^27 Zipf 0.5 hash and sort-merge joins, and the
+batched workload of 4096 mixed joins (--no-batched skips it).
 
---extra adds the skewed 2^26 x 2^28 runs (Zipf 0 / 1.0 / 1.5) and the
-sort-merge variant of the default workload to the same line.
+--headline-only runs the default workload alone (for profiling);
+--batched and --serving print their own lines instead.
 """
 import argparse
 import json
@@ -32,7 +36,9 @@ def log(msg):
 
 
 def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
-    """Generate, upload, warm up, then time `steps` joins.  Returns a dict."""
+    """Generate, upload, warm up, then time `steps` joins.  Each timed join
+    records HIP events between its phases on the join's own stream; the
+    per-phase times are averaged over the timed joins.  Returns a dict."""
     t0 = time.time()
     if keep is not None and keep.get("key") == (nR, seed):
         R = keep["R"]
@@ -44,9 +50,18 @@ def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
         dR = sj.DeviceArray.from_numpy(R)
         if keep is not None:
             keep.update(key=(nR, seed), R=R, dR=dR)
-    S = sj.gen_s(nS, R, zipf, seed + 1)
-    dS = sj.DeviceArray.from_numpy(S)
-    del S
+    skey = (nS, zipf, seed)
+    if keep is not None and keep.get("skey") == skey:
+        dS = keep["dS"]
+    else:
+        if keep is not None and keep.get("dS") is not None:
+            keep["dS"].free()
+            keep["dS"] = None
+        S = sj.gen_s(nS, R, zipf, seed + 1 + int(zipf * 100))
+        dS = sj.DeviceArray.from_numpy(S)
+        del S
+        if keep is not None:
+            keep.update(skey=skey, dS=dS)
     dO = sj.DeviceArray(nS * 8)
     log(f"[bench] nR={nR} nS={nS} zipf={zipf}: data ready in {time.time() - t0:.1f}s")
     try:
@@ -55,18 +70,18 @@ def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
             if n != nS:
                 raise RuntimeError(f"join produced {n} pairs, expected {nS}")
         sj.check(sj.lib().spj_dev_sync())
+        phases = []
         t1 = time.perf_counter()
         for _ in range(steps):
-            n, _t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=False)
+            n, t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)
+            phases.append(t)
+            if n != nS:
+                raise RuntimeError(f"join produced {n} pairs, expected {nS}")
         sj.check(sj.lib().spj_dev_sync())
         wall = time.perf_counter() - t1
-        if n != nS:
-            raise RuntimeError(f"join produced {n} pairs, expected {nS}")
-        # per-phase breakdown: HIP events between the phases, in extra joins
-        # after the timed region (the events themselves cost ~2% of a step)
-        phases = [sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)[1] for _ in range(3)]
     finally:
-        dS.free()
+        if keep is None:
+            dS.free()
         dO.free()
     ms = wall / steps * 1e3
     avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
@@ -74,7 +89,7 @@ def run_workload(nR, nS, zipf, steps, warmup, cfg, seed, keep=None):
     return {"nR": nR, "nS": nS, "zipf": zipf, "ms_per_step": ms, "phases": avg, "pairs": int(n)}
 
 
-def roofline(res):
+def roofline(res, workload):
     """Dominant kernel and its achieved algorithmic GB/s.
 
     Algorithmic bytes per join (the phase's launches together: the scatter
@@ -100,20 +115,27 @@ def roofline(res):
     ach = byts / (ms * 1e-3)
     b_alg = 24 * (nR + nS) + 8 * res["pairs"]
     return name, {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(name)}, {
+                  "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(name, workload)}, {
         k: {"ms": round(v[0], 4), "alg_bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)}
         for k, v in kern.items()}, b_alg
 
 
-def measured_traffic(kernel):
+TRAFFIC_FILE = "c3_traffic.json"   # counters of the default workload
+
+
+def measured_traffic(kernel, workload):
     """HBM bytes of `kernel` per join (FETCH_SIZE + WRITE_SIZE over its
     dispatches of one join, scaled by the plain-copy calibration) from the
     committed rocprofv3 counter summary of this same workload
-    (profiles/c2_traffic.json), or None."""
-    path = os.path.join(ROOT, "profiles", "c2_traffic.json")
+    (profiles/c3_traffic.json), or None when the summary is of another
+    workload."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
     try:
         with open(path) as f:
-            per = json.load(f)["per_join"]
+            doc = json.load(f)
+        if doc.get("workload") != workload:
+            return None
+        per = doc["per_join"]
         # template instances carry their argument (k_scatter<12>)
         key = next(k for k in per if k == kernel or k.startswith(kernel + "<"))
         t = per[key]
@@ -122,25 +144,26 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(zipf, seconds=10.0):
+def cpu_baseline(seconds=10.0):
     """The oracle's CPU restatement of the reference join (global SFH partition
     into 1024 virtual DPUs, P_loc local partitions, 4096-slot linear probing)
-    on a bounded sample of the same workload, on the host cores."""
+    on the reference's CPU-runnable workload: |R| = 2^20 (permutation of
+    1..2^20), |S| = 2^21 uniform over R's keys, on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    nR, nS = 1 << 22, 1 << 23   # 1/16 of the default workload, same R:S ratio and skew
+    nR, nS = 1 << 20, 1 << 21
     R = sj.gen_r(nR, 7)
-    S = sj.gen_s(nS, R, zipf, 8)
+    S = sj.gen_s(nS, R, 0.0, 8)
     oracle.spid_join_cpu(R, S, threads=threads)   # warm-up
     ts = []
     t_end = time.time() + seconds
-    while time.time() < t_end or len(ts) < 3:
+    while time.time() < t_end or len(ts) < 5:
         t = time.perf_counter()
         out = oracle.spid_join_cpu(R, S, threads=threads)
         ts.append(time.perf_counter() - t)
-        if len(ts) >= 15:
+        if len(ts) >= 200:
             break
     if len(out) != nS:
         raise RuntimeError("CPU baseline produced a wrong count")
@@ -155,7 +178,8 @@ def cpu_baseline(zipf, seconds=10.0):
     except OSError:
         pass
     return {"value": round((nR + nS) / med, 1), "unit": "joined tuples/s", "cores": threads, "kind": "port",
-            "sample": f"|R|=2^22, |S|=2^23, Zipf {zipf}, median of {len(ts)} runs ({med * 1e3:.1f} ms each); {cpu}"}
+            "sample": f"|R|=2^20, |S|=2^21, Zipf 0 (the reference's CPU-runnable workload), median of {len(ts)} "
+                      f"runs ({med * 1e3:.2f} ms each); {cpu}"}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -174,15 +198,15 @@ def copy_calibration(nbytes=1 << 30, reps=3):
                                                                               for m in out]}), flush=True)
 
 
-def batched(args):
-    """Batched mixed joins: instances with |R| log-uniform in [2^10, 2^max_log2r],
-    R:S from {1,2,4,8}, Zipf U[0, 1.25].  One batched launch sequence
-    (persistent join kernel over one work queue) against one join per call,
-    with identical per-instance results (count and pair-multiset checksum)."""
+def batched_run(n, max_log2r, steps, warmup):
+    """Batched mixed joins: n instances with |R| log-uniform in
+    [2^10, 2^max_log2r], R:S from {1,2,4,8}, Zipf U[0, 1.25].  One batched
+    launch sequence (persistent join kernel over one work queue) against one
+    join per call, with identical per-instance results (count and
+    pair-multiset checksum).  Returns a dict."""
     import ctypes
     rng = np.random.default_rng(4096)
-    n = args.instances
-    nRs = np.floor(2.0 ** rng.uniform(10, args.max_log2r, n)).astype(np.int64)
+    nRs = np.floor(2.0 ** rng.uniform(10, max_log2r, n)).astype(np.int64)
     ratios = rng.choice([1, 2, 4, 8], n)
     zipfs = rng.uniform(0, 1.25, n)
     nSs = nRs * ratios
@@ -215,56 +239,73 @@ def batched(args):
     def checksums():
         return [sj.checksum_device(int(op[i]), int(cnt[i])) for i in range(n)]
 
-    for _ in range(args.warmup):
-        run_batch()
-    sj.check(L.spj_dev_sync())
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        run_batch()
-    sj.check(L.spj_dev_sync())
-    t_batch = (time.perf_counter() - t1) / args.steps
-    counts_b = [int(cnt[i]) for i in range(n)]
-    sums_b = checksums()
-    # one join per call
-    lat = []
-    one = sj.lib().spj_join_device
-    c1 = ctypes.c_uint64()
-    for i in range(min(n, 16)):   # warm-up
-        sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
-    sj.check(L.spj_dev_sync())
-    counts_1 = []
-    t2 = time.perf_counter()
-    for i in range(n):
-        ta = time.perf_counter()
-        sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
-        lat.append(time.perf_counter() - ta)
-        counts_1.append(int(c1.value))
-    t_single = time.perf_counter() - t2
-    for i in range(n):
-        cnt[i] = counts_1[i]
-    sums_1 = checksums()
+    try:
+        for _ in range(warmup):
+            run_batch()
+        sj.check(L.spj_dev_sync())
+        blat = []
+        t1 = time.perf_counter()
+        for _ in range(steps):   # each call returns when its batch is done
+            ta = time.perf_counter()
+            run_batch()
+            blat.append(time.perf_counter() - ta)
+        sj.check(L.spj_dev_sync())
+        t_batch = (time.perf_counter() - t1) / steps
+        counts_b = [int(cnt[i]) for i in range(n)]
+        sums_b = checksums()
+        # one join per call
+        lat = []
+        one = sj.lib().spj_join_device
+        c1 = ctypes.c_uint64()
+        for i in range(min(n, 16)):   # warm-up
+            sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
+        sj.check(L.spj_dev_sync())
+        counts_1 = []
+        t2 = time.perf_counter()
+        for i in range(n):
+            ta = time.perf_counter()
+            sj.check(one(rp[i], nr[i], sp[i], ns[i], op[i], ns[i], ctypes.byref(c1), None, None))
+            lat.append(time.perf_counter() - ta)
+            counts_1.append(int(c1.value))
+        t_single = time.perf_counter() - t2
+        for i in range(n):
+            cnt[i] = counts_1[i]
+        sums_1 = checksums()
+    finally:
+        dR.free()
+        dS.free()
+        dO.free()
     identical = counts_b == counts_1 and sums_b == sums_1 and counts_b == [int(x) for x in nSs]
     lat_ms = np.array(lat) * 1e3
-    dR.free()
-    dS.free()
-    dO.free()
+    blat_ms = np.array(blat) * 1e3
     tuples = totR + totS
-    out = {
-        "metric": "batched joined tuples/s (sum |R|+|S| over instances)",
-        "value": round(tuples / t_batch, 1), "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "uint32",
-        "data": "synthetic (seeded: |R| log-uniform, R:S in {1,2,4,8}, Zipf U[0,1.25])",
-        "config": {"workload": f"batched mixed joins, {n} instances, |R| in [2^10, 2^{args.max_log2r}]",
-                   "instances": n, "sum_R": totR, "sum_S": totS},
+    return {
+        "instances": n, "max_log2r": max_log2r, "sum_R": totR, "sum_S": totS,
+        "batch_ms": round(t_batch * 1e3, 3), "batch_tuples_per_s": round(tuples / t_batch, 1),
+        "batch_latency_ms_p50": round(float(np.percentile(blat_ms, 50)), 3),
+        "batch_latency_ms_p99": round(float(np.percentile(blat_ms, 99)), 3),
         "one_join_per_call": {"tuples_per_s": round(tuples / t_single, 1), "total_ms": round(t_single * 1e3, 2),
                               "latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 4),
                               "latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 4)},
         "speedup_batched_over_one_per_call": round(t_single / t_batch, 2),
         "identical_per_instance_results": bool(identical),
     }
+
+
+def batched(args):
+    r = batched_run(args.instances, args.max_log2r, args.steps, args.warmup)
+    out = {
+        "metric": "batched joined tuples/s (sum |R|+|S| over instances)",
+        "value": r["batch_tuples_per_s"], "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["batch_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "uint32",
+        "data": "synthetic (seeded: |R| log-uniform, R:S in {1,2,4,8}, Zipf U[0,1.25])",
+        "config": {"workload": f"batched mixed joins, {r['instances']} instances, |R| in [2^10, 2^{args.max_log2r}]",
+                   "instances": r["instances"], "sum_R": r["sum_R"], "sum_S": r["sum_S"]},
+        "batched": r,
+    }
     print(json.dumps(out), flush=True)
-    if not identical:
+    if not r["identical_per_instance_results"]:
         sys.exit(1)
 
 
@@ -350,7 +391,9 @@ def main():
     # the GPU's clocks take ~8 joins of sustained load to settle (per-call
     # times fall ~7% over them), so the default warm-up covers that
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--extra", action="store_true", help="also run the skewed 2^26 x 2^28 and sort-merge runs")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the default workload (2^26 x 2^28, Zipf 0), e.g. under a profiler")
+    ap.add_argument("--no-batched", action="store_true", help="leave the batched workload out of the default line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device-to-device copies (the plain copy kernel that scales the "
@@ -381,42 +424,58 @@ def main():
     if args.serving:
         serving(args)
         return
-    nR, nS, zipf = 1 << 26, 1 << 27, 0.5
+    nR, nS, zipf = 1 << 26, 1 << 28, 0.0
+    workload = "hash equi-join |R|=2^26 x |S|=2^28, Zipf 0, inputs resident in HBM"
     keep = {}
-    res = run_workload(nR, nS, zipf, args.steps, args.warmup, None, 1000, keep)
-    kname, roof, kernels, b_alg = roofline(res)
-    ms = res["ms_per_step"]
-    value = (nR + nS) / (ms * 1e-3)
-    b_min = 8 * (nR + nS) + 8 * res["pairs"]   # compulsory: read R and S once, write the pairs
-    measured = {}
-    for k in ("k_scatter", "k_join"):   # rocprof HBM-side bytes per join over the phase's HIP-event time
-        tb = measured_traffic(k)
-        if tb is not None and kernels[k]["ms"] > 0:
-            measured[k] = {"bytes": tb, "GB/s": round(tb / (kernels[k]["ms"] * 1e-3) / 1e9, 1)}
-    extra = {"kernels": kernels, "dominant_kernel": kname,
-             "join_B_alg_bytes": b_alg,
-             "join_roofline_frac": round(b_alg / (ms * 1e-3) / HBM_PEAK, 4),
-             "join_B_min_bytes": b_min,
-             "join_B_min_frac": round(b_min / (ms * 1e-3) / HBM_PEAK, 4),
-             "measured_traffic": measured,
-             "phases_ms": {k: round(v, 4) for k, v in res["phases"].items()}}
-    if args.extra:
-        smj = run_workload(nR, nS, zipf, max(3, args.steps // 4), 2, sj.config(algorithm=sj.ALG_SMJ), 1000, keep)
-        extra["smj"] = {"ms_per_step": round(smj["ms_per_step"], 4),
-                        "tuples_per_s": round((nR + nS) / (smj["ms_per_step"] * 1e-3), 1)}
-        sk = {}
-        for z in (0.0, 1.0, 1.5):
-            r = run_workload(1 << 26, 1 << 28, z, max(10, args.steps // 2), args.warmup, None, 2000, keep)
-            bal = 24 * ((1 << 26) + (1 << 28)) + 8 * r["pairs"]
-            sk[str(z)] = {"ms_per_step": round(r["ms_per_step"], 4),
-                          "tuples_per_s": round(((1 << 26) + (1 << 28)) / (r["ms_per_step"] * 1e-3), 1),
-                          "roofline_frac": round(bal / (r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
-                          "heavy_keys": r["phases"]["heavy_keys"]}
-        sk["zipf1.5_over_zipf0"] = round(sk["1.5"]["ms_per_step"] / sk["0.0"]["ms_per_step"], 3)
-        extra["skewed_2^26x2^28"] = sk
-    if keep.get("dR") is not None:
-        keep["dR"].free()
-    cpu = None if args.no_cpu else cpu_baseline(zipf)
+    try:
+        res = run_workload(nR, nS, zipf, args.steps, args.warmup, None, 1000, keep)
+        kname, roof, kernels, b_alg = roofline(res, workload)
+        ms = res["ms_per_step"]
+        value = (nR + nS) / (ms * 1e-3)
+        b_min = 8 * (nR + nS) + 8 * res["pairs"]   # compulsory: read R and S once, write the pairs
+        measured = {}
+        for k in ("k_scatter", "k_join"):   # rocprof HBM-side bytes per join over the phase's event time
+            tb = measured_traffic(k, workload)
+            if tb is not None and kernels[k]["ms"] > 0:
+                measured[k] = {"bytes": tb, "GB/s": round(tb / (kernels[k]["ms"] * 1e-3) / 1e9, 1)}
+        extra = {"kernels": kernels, "dominant_kernel": kname,
+                 "join_B_alg_bytes": b_alg,
+                 "join_roofline_frac": round(b_alg / (ms * 1e-3) / HBM_PEAK, 4),
+                 "join_B_min_bytes": b_min,
+                 "join_B_min_frac": round(b_min / (ms * 1e-3) / HBM_PEAK, 4),
+                 "measured_traffic": measured,
+                 "phases_ms": {k: round(v, 4) for k, v in res["phases"].items()}}
+        if not args.headline_only:
+            sk = {"0.0": {"ms_per_step": round(ms, 4), "roofline_frac": extra["join_roofline_frac"]}}
+            for z in (1.0, 1.5):
+                r = run_workload(nR, nS, z, args.steps, args.warmup, None, 1000, keep)
+                bal = 24 * (nR + nS) + 8 * r["pairs"]
+                _, rz, kz, _ = roofline(r, None)
+                sk[str(z)] = {"ms_per_step": round(r["ms_per_step"], 4),
+                              "tuples_per_s": round((nR + nS) / (r["ms_per_step"] * 1e-3), 1),
+                              "roofline_frac": round(bal / (r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
+                              "heavy_keys": r["phases"]["heavy_keys"],
+                              "s_partitioned": r["phases"]["s_partitioned"],
+                              "kernels": kz}
+            sk["zipf1.5_over_zipf0"] = round(sk["1.5"]["ms_per_step"] / ms, 3)
+            sk["zipf1.0_over_zipf0"] = round(sk["1.0"]["ms_per_step"] / ms, 3)
+            extra["skew_2^26x2^28"] = sk
+            c2 = {}
+            for name, cfg in (("hash", None), ("sort_merge", sj.config(algorithm=sj.ALG_SMJ))):
+                r = run_workload(nR, 1 << 27, 0.5, max(5, args.steps // 2), args.warmup, cfg, 1000, keep)
+                _, _, kz, bal = roofline(r, None)
+                c2[name] = {"ms_per_step": round(r["ms_per_step"], 4),
+                            "tuples_per_s": round((nR + (1 << 27)) / (r["ms_per_step"] * 1e-3), 1),
+                            "roofline_frac": round(bal / (r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
+                            "kernels": kz}
+            extra["c2_2^26x2^27_zipf0.5"] = c2
+    finally:
+        for k in ("dR", "dS"):
+            if keep.get(k) is not None:
+                keep[k].free()
+    if not (args.headline_only or args.no_batched):
+        extra["batched_4096_mixed"] = batched_run(4096, 20.0, max(5, args.steps // 2), 2)
+    cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "joined tuples/s (|R|+|S|)",
         "value": round(value, 1),
@@ -429,9 +488,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "uint32",
-        "data": "synthetic (seeded generator: R = permutation of 1..2^26, S Zipf 0.5 over R's keys)",
-        "config": {"workload": "hash equi-join |R|=2^26 x |S|=2^27, Zipf 0.5, inputs resident in HBM",
-                   "nR": nR, "nS": nS, "zipf": zipf, "algorithm": "radix-partitioned hash join"},
+        "data": "synthetic (seeded generator: R = permutation of 1..2^26, S uniform over R's keys)",
+        "config": {"workload": workload, "nR": nR, "nS": nS, "zipf": zipf, "algorithm": "radix-partitioned hash join"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,
