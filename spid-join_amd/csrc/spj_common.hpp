@@ -28,8 +28,8 @@ namespace spj {
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kEmptyKey = 0u;          // key 0 = empty / skipped (glb_partition_count.c:192)
 constexpr uint32_t kHeavySlots = 4096;      // heavy (key -> R rid) table, LDS resident
-// One instance's heavy table in device memory: kHeavySlots keys (0 = empty),
-// kHeavySlots R rids, then a presence bitmap (bit s set: slot s's key has an
+// One instance's heavy table in device memory (and in LDS, in the same
+// order): kHeavySlots keys (0 = empty), kHeavySlots R rids, then a presence bitmap (bit s set: slot s's key has an
 // R tuple and its rid is valid).  Presence is kept apart from the rid because
 // every 32-bit value is a legal rid (kv_pair_t.rvalue is a full uint32).
 constexpr uint32_t kHeavyPresWords = kHeavySlots / 32;

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "spj_common.hpp"
 #include "spj_kernels.hpp"
@@ -113,8 +114,14 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 // the total.  wtmp needs blockDim/64 + 1 words.
 __device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wtmp)
 {
-    const uint32_t nt = blockDim.x, tid = threadIdx.x;
-    const uint32_t nw = nt / kWave, wid = tid / kWave, lane = lane_id();
+    // The thread index is made opaque, so that the lane masks and shuffle
+    // addresses derived from it are formed at each call: hoisted out of a
+    // caller's loop they would hold registers for the whole loop (and spill
+    // in the scatter, whose reloads wait behind the round's stores).
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t nt = blockDim.x;
+    const uint32_t nw = nt / kWave, wid = tid / kWave, lane = tid & (kWave - 1);
     uint32_t per = (n + nt - 1) / nt;
     uint32_t b = tid * per, e = min(n, b + per);
     uint32_t s = 0;
@@ -165,15 +172,16 @@ __device__ __forceinline__ uint32_t heavy_keys_of(const InstDesc &I)
     return *reinterpret_cast<const_u32 *>(reinterpret_cast<uintptr_t>(I.heavy_count));
 }
 
-// Load the instance's heavy table into LDS: the keys, and with rids set
-// the R rids and the presence bitmap behind them (kHeavyWords - kHeavySlots
-// words at hr).  The caller synchronises before using it.
-__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t *hr, bool rids)
+// Load the first n words of the instance's heavy table into LDS at hk: the
+// keys (n = kHeavySlots), or the keys, R rids and presence bitmap, which lie
+// back to back in both places (n = kHeavyWords).  One flat loop: the table
+// is loaded inside the scatter's round loop, whose registers are tight.  The
+// caller synchronises before using it.
+__device__ __forceinline__ void load_heavy(const InstDesc &I, uint32_t *hk, uint32_t n)
 {
-    for (uint32_t i = threadIdx.x; i < kHeavySlots; i += blockDim.x) hk[i] = gld32(I.heavy_keys, i);
-    if (rids)
-        for (uint32_t i = threadIdx.x; i < kHeavySlots + kHeavyPresWords; i += blockDim.x)
-            hr[i] = gld32(I.heavy_rids, i);
+    uint32_t i0 = threadIdx.x;   // opaque: the addresses are formed here, not held across the caller's loop
+    asm volatile("" : "+v"(i0));
+    for (uint32_t i = i0; i < n; i += blockDim.x) hk[i] = gld32(I.heavy_keys, i);
 }
 
 // R side: slot s's key has an R tuple with this rid.  R keys are unique, so
@@ -283,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
     uint32_t *hist = hk + kHeavySlots;    // F
     const bool heavy = I.heavy != 0 && heavy_keys_of(I) != 0;
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) hist[i] = 0;
-    if (heavy) load_heavy(I, hk, nullptr, false);
+    if (heavy) load_heavy(I, hk, kHeavySlots);
     __syncthreads();
     const uint2 *in = td.rel ? I.S : I.R;
     const uint32_t mode = !heavy ? 0u : (td.rel == 0 ? 1u : 2u);
@@ -432,11 +440,13 @@ __device__ __forceinline__ void scatter_write_out(const uint2 *stage, const uint
                                                   uint32_t hstart, uint32_t total, uint2 *outp, uint2 *out,
                                                   uint32_t hb, uint32_t nS, uint32_t *err_inst)
 {
-    for (uint32_t i = threadIdx.x; i < hstart; i += kScatterNT) {
+    uint32_t i0 = threadIdx.x;   // opaque: the stage addresses are formed here, not held across the round loop
+    asm volatile("" : "+v"(i0));
+    for (uint32_t i = i0; i < hstart; i += kScatterNT) {
         const uint2 t = stage[i];
         gst(outp, bs[sfh(t.x) & mask] + i, t);
     }
-    for (uint32_t i = hstart + threadIdx.x; i < total; i += kScatterNT) {
+    for (uint32_t i = hstart + i0; i < total; i += kScatterNT) {
         const uint32_t oi = hb + (i - hstart);
         if (oi < nS) gst(out, oi, stage[i]);
         else atomicOr(err_inst, kErrOutOverflow);
@@ -481,8 +491,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     if (t >= te) return;   // whole workgroup
     __syncthreads();   // misc[0] and misc[1] read before reuse
     if (threadIdx.x == 0) misc[1] = misc[2];
-    uint32_t qv = 0;
-    bool qpending = false;   // thread 0: a queue atomic whose value goes to misc[1]
+    bool qwant = false;   // thread 0: take the next tile from the queue in this round (its value goes to misc[1])
     TileDesc td = tiles[t];
     uint32_t o = td.begin;
     uint32_t heavy_inst = DROP;   // instance whose heavy table is in LDS
@@ -517,7 +526,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         const uint32_t rn = min(ROUND, td.end - o);
         __syncthreads();   // the previous round's write-out and clearing are done
         if ((heavy || rec) && heavy_inst != td.inst) {
-            load_heavy(I, hk, hr, true);
+            load_heavy(I, hk, kHeavyWords);
             heavy_inst = td.inst;
             __syncthreads();
         }
@@ -532,14 +541,14 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 const uint32_t li = j * NT + threadIdx.x;
                 hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
             }
-            uint32_t hrid[IPT];
-            bool hpres[IPT];
+            // a heavy key with no R tuple: the tuple is dropped, as one
+            // with the empty key 0 (it was not counted either)
 #pragma unroll
-            for (int j = 0; j < IPT; j++) {
-                hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : 0u;
-                hpres[j] = (hs[j] >= 0) && heavy_present(hr, hs[j]);
-            }
-            uint64_t hm[IPT];
+            for (int j = 0; j < IPT; j++)
+                if (hs[j] >= 0 && !heavy_present(hr, hs[j])) v[j].x = kEmptyKey;
+            uint32_t hrid[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; j++) hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : 0u;
             uint32_t htot = 0;
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
@@ -547,20 +556,21 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 dg[j] = DROP;
                 if (li < rn && v[j].x != kEmptyKey) {
                     if (hs[j] < 0) dg[j] = sfh(v[j].x) & mask;
-                    else if (hpres[j]) { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
-                    // (a heavy key with no R tuple is dropped)
+                    else { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
                 }
-                hm[j] = __ballot(dg[j] == F);
-                htot += (uint32_t)__popcll(hm[j]);
+                htot += (uint32_t)__popcll(__ballot(dg[j] == F));
             }
             if (htot) {
                 uint32_t hb = 0;
                 if (lane_id() == 0) hb = atomicAdd(&lst[F], htot);
                 hb = __shfl(hb, 0, kWave);
+                // the ballots again (cheaper than holding twelve 64-bit
+                // masks in registers across the atomic)
 #pragma unroll
                 for (int j = 0; j < IPT; j++) {
-                    if (dg[j] == F) rk[j] = hb + mbcnt(hm[j]);
-                    hb += (uint32_t)__popcll(hm[j]);
+                    const uint64_t hm = __ballot(dg[j] == F);
+                    if (dg[j] == F) rk[j] = hb + mbcnt(hm);
+                    hb += (uint32_t)__popcll(hm);
                 }
             }
         } else {
@@ -593,6 +603,11 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             const uint32_t hn = lst[F];
             if (hn) hres = atomicAdd(&out_hi[td.inst], hn);
         }
+        // the tile queue's next value, like the reservations: issued here,
+        // its result due at the wait after the staging (one round after the
+        // tile switch that asked for it, before the next switch reads it)
+        uint32_t qv = 0;
+        if (qwant) qv = atomicAdd(q, 1u);
         uint32_t cb[FMAX / NT], cc[FMAX / NT];
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
@@ -619,9 +634,9 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // at the next round's start and does not wait there -- a wait that
         // would also cover this round's stores, issued after them.
         __builtin_amdgcn_s_waitcnt(0);
-        if (qpending) {   // thread 0; the atomic completed with the wait above
+        if (qwant) {   // thread 0; the atomic completed with the wait above
             misc[1] = qv;
-            qpending = false;
+            qwant = false;
         }
         __syncthreads();   // staging has read the starts; lst[d] becomes digit d's base - start
 #pragma unroll
@@ -647,14 +662,15 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         scatter_write_out(stage, bs, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
                           &err[td.inst]);
         __syncthreads();   // the write-out has read the bases
-        for (uint32_t i = threadIdx.x; i < F + 2; i += NT) lst[i] = 0;
+        {
+            uint32_t i0 = threadIdx.x;   // opaque, as in the write-out
+            asm volatile("" : "+v"(i0));
+            for (uint32_t i = i0; i < F + 2; i += NT) lst[i] = 0;
+        }
         if (!more) break;
         if (tn != t) {   // next tile: the one after it was taken a round ago at least
             tnext = x + kScatterGroups * misc[1];
-            if (threadIdx.x == 0) {
-                qv = atomicAdd(q, 1u);
-                qpending = true;
-            }
+            qwant = threadIdx.x == 0;
         }
         t = tn;
         td = tdn;
@@ -1851,10 +1867,29 @@ __global__ __launch_bounds__(256) void k_pairs_checksum(const uint2 *__restrict_
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// Allow `bytes` of dynamic LDS for kernel fn on the current device.  The
+// attribute is set once per (device, kernel) and raised only when a launch
+// asks for more: it is a runtime call of several microseconds, which every
+// launch would otherwise pay while the GPU waits.
 static hipError_t set_lds(const void *fn, size_t bytes)
 {
     if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<int, const void *>, size_t>> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &d : done)
+        if (d.first.first == dev && d.first.second == fn) {
+            if (d.second >= bytes) return hipSuccess;
+            e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e == hipSuccess) d.second = bytes;
+            return e;
+        }
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) done.push_back({{dev, fn}, bytes});
+    return e;
 }
 
 size_t lds_hist(uint32_t max_bits)
