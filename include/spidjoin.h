@@ -98,6 +98,15 @@ int spj_dev_memset(void *ptr, int value, uint64_t bytes);
  * (used to calibrate achievable HBM bandwidth). */
 int spj_d2d(void *dst, const void *src, uint64_t bytes, float *ms);
 int spj_dev_sync(void);
+/* Launch sequences of the device-buffer calls (spj_join_device,
+ * spj_join_batch_device) replayed as HIP graphs: a call that repeats the
+ * previous call exactly (same pointers, sizes and parameters) replays the
+ * previous call's kernels and copies as one captured graph instead of
+ * launching them one by one; any other call launches directly.  Timed calls
+ * (times != NULL) always launch directly.  The environment variable
+ * SPJ_GRAPHS=0 turns replays off.  No counterpart in the reference; these
+ * counters say how often a graph was captured and replayed. */
+int spj_graph_stats(uint64_t *captures, uint64_t *replays);
 /* Order-independent checksum of device-resident pairs: sum and xor of a
  * 64-bit mix of (r_rid << 32 | s_rid) -- compares result multisets in place. */
 int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum, uint64_t *xr);

@@ -242,6 +242,16 @@ int spj_pairs_checksum_device(const spj_pair_t *pairs, uint64_t n, uint64_t *sum
     return SPJ_OK;
 }
 
+int spj_graph_stats(uint64_t *captures, uint64_t *replays)
+{
+    if (!captures || !replays) return fail(SPJ_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(default_context_mutex());
+    Context &c = default_context();
+    *captures = c.graph_captures();
+    *replays = c.graph_replays();
+    return SPJ_OK;
+}
+
 int spj_dev_sync(void)
 {
     hipError_t e = hipDeviceSynchronize();

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "spj_kernels.hpp"
@@ -129,6 +130,8 @@ Resolved default_params(uint32_t nR, uint32_t nS)
 
 Context::Context()
 {
+    const char *g = getenv("SPJ_GRAPHS");   // SPJ_GRAPHS=0: never replay launch sequences as graphs
+    graphs_ = !(g && g[0] == '0');
     (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     for (auto &e : ev_) (void)hipEventCreate(&e);
     (void)hipEventCreateWithFlags(&desc_ev_, hipEventDisableTiming);
@@ -141,6 +144,7 @@ Context::Context()
 Context::~Context()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
@@ -176,7 +180,79 @@ static uint32_t cell_capacity(uint64_t n, uint32_t F, uint32_t pct)
     return (uint32_t)std::ceil(1.5 * m + 8.0 * std::sqrt(m) + 64.0);
 }
 
+static bool same_params(const Resolved &a, const Resolved &b)
+{
+    return a.bits == b.bits && a.table_log2 == b.table_log2 && a.alg == b.alg && a.heavy == b.heavy &&
+           a.heavy_min == b.heavy_min && a.ipt == b.ipt && a.tile == b.tile && a.smax == b.smax && a.cells == b.cells &&
+           a.cell_pct == b.cell_pct;
+}
+
+static bool same_specs(const std::vector<InstSpec> &a, const std::vector<InstSpec> &b)
+{
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++) {
+        const InstSpec &x = a[i], &y = b[i];
+        if (x.R != y.R || x.nR != y.nR || x.S != y.S || x.nS != y.nS || x.out != y.out || x.Rp_pre != y.Rp_pre ||
+            x.pst_pre != y.pst_pre || x.r_contiguous != y.r_contiguous || !same_params(x.cfg, y.cfg))
+            return false;
+    }
+    return true;
+}
+
+void Context::drop_graph()
+{
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+}
+
+// A call that repeats the previous call's specs exactly (same pointers,
+// sizes and parameters) replays the launch sequence as a HIP graph: the
+// planning, the descriptor contents and every buffer address are the same,
+// so only the host's launch work is saved.  The graph is captured at the
+// first repeat and dropped by any other call (which may move buffers).
+// Timed calls (phase events) and the counted redo always launch directly.
 hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *timed)
+{
+    if (graphs_ && !timed && !force_counted_ && last_valid_ && &specs_in != &h_specs_ && same_specs(specs_in, h_specs_)) {
+        if (!graph_exec_) {
+            // the staging of the descriptors must be free before the capture
+            // rewrites it (the graph's copy node reads it at every replay)
+            if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));
+            desc_pending_ = false;
+            hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                capturing_ = true;
+                e = enqueue_plan(h_specs_, nullptr);
+                capturing_ = false;
+                hipGraph_t g = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+                if (e == hipSuccess) e = e2;
+                if (e == hipSuccess) e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) graph_exec_ = nullptr;
+                else graph_captures_++;
+            }
+            if (e != hipSuccess) {   // no graph for this call: launch directly, and stop trying
+                (void)hipGetLastError();
+                graphs_ = false;
+                last_valid_ = false;
+                return enqueue(specs_in, timed);
+            }
+        }
+        SPJ_TRY(hipGraphLaunch(graph_exec_, stream_));
+        graph_replays_++;
+        SPJ_TRY(hipEventRecord(desc_ev_, stream_));   // the replay's copy has read the staging
+        desc_pending_ = true;
+        return hipSuccess;
+    }
+    drop_graph();
+    last_valid_ = false;
+    const hipError_t e = enqueue_plan(specs_in, timed);
+    last_valid_ = e == hipSuccess && !force_counted_;
+    return e;
+}
+
+hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTimes *timed)
 {
     if (&specs_in != &h_specs_) h_specs_ = specs_in;
     const std::vector<InstSpec> &specs = h_specs_;
@@ -453,7 +529,7 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
     hipStream_t st = stream_;
     // one upload of all descriptors through page-locked staging; the
     // staging is reused once the previous upload has left it
-    if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));
+    if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));   // (never pending while capturing)
     SPJ_TRY(pin_desc_.ensure(desc_bytes));
     {
         char *h = pin_desc_.as<char>();
@@ -467,8 +543,10 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         if (!h_nphj_.empty()) memcpy(h + o_nphj, h_nphj_.data(), h_nphj_.size() * sizeof(NphjJob));
     }
     SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
-    SPJ_TRY(hipEventRecord(desc_ev_, st));
-    desc_pending_ = true;
+    if (!capturing_) {   // a replay records it after its launch
+        SPJ_TRY(hipEventRecord(desc_ev_, st));
+        desc_pending_ = true;
+    }
     char *dd = desc_.as<char>();
     const TileRun *d_runs = reinterpret_cast<const TileRun *>(dd + o_runs);
     const TileDesc *d_tiles = tiles_.as<TileDesc>();

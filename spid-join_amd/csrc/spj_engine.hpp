@@ -100,12 +100,24 @@ public:
     hipError_t enqueue(const std::vector<InstSpec> &specs, PhaseTimes *timed);
     hipError_t collect(size_t ninst, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed);
 
+    // Launch sequences replayed as HIP graphs: graphs captured, replays.
+    uint64_t graph_captures() const { return graph_captures_; }
+    uint64_t graph_replays() const { return graph_replays_; }
+
     // After run()/collect() of a batch, copy instance i's partitioned R and
     // its F + 1 partition starts to caller-owned device buffers.
     hipError_t export_partitioned_r(size_t inst, uint2 *rp_dst, uint32_t *pstart_dst);
 
 private:
+    hipError_t enqueue_plan(const std::vector<InstSpec> &specs, PhaseTimes *timed);
+    void drop_graph();
     hipStream_t stream_ = nullptr;
+    // graph replay of a repeated call (see enqueue)
+    bool graphs_ = true;            // off after a failed capture, or with SPJ_GRAPHS=0
+    bool last_valid_ = false;       // h_specs_ is the last call, launched directly, buffers unmoved since
+    bool capturing_ = false;
+    hipGraphExec_t graph_exec_ = nullptr;
+    uint64_t graph_captures_ = 0, graph_replays_ = 0;
     hipEvent_t ev_[8] = {};   // [0, 6]: phase boundaries; [7]: end of R's own scatter launch
     bool split_timed_ = false;  // the last timed enqueue recorded ev_[7]
     // desc_: every descriptor array of a batch (instances, tiles, scan

@@ -24,7 +24,7 @@ ERRORS = {0: "OK", -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EDUPKEY", -5: "E
 EXPORTS = [
     "spj_config_default", "spj_last_error", "spj_init", "spj_device_info", "spj_dev_alloc", "spj_dev_free",
     "spj_h2d", "spj_d2h", "spj_dev_memset", "spj_d2d", "spj_pairs_checksum_device", "spj_dev_sync", "spj_join_device",
-    "spj_partition_device", "spj_join_device_noalloc", "spj_join_host",
+    "spj_partition_device", "spj_join_device_noalloc", "spj_graph_stats", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_set_query_plan", "spj_plan_check",
     "spj_timeline", "spj_free_result", "spj_destroy",
@@ -95,6 +95,7 @@ def lib():
                                            ctypes.POINTER(PhaseTimes)]),
         "spj_partition_device": (ctypes.c_int, [P, U64, ctypes.c_int, P, P]),
         "spj_join_device_noalloc": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64)]),
+        "spj_graph_stats": (ctypes.c_int, [ctypes.POINTER(U64), ctypes.POINTER(U64)]),
         "spj_join_host": (ctypes.c_int, [P, U64, P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(Config)]),
         "spj_join_batch_device": (ctypes.c_int, [ctypes.c_int, P, P, P, P, P, P, ctypes.POINTER(Config),
                                                  ctypes.POINTER(PhaseTimes)]),
@@ -212,6 +213,13 @@ def checksum_device(ptr, n):
     s, x = U64(), U64()
     check(lib().spj_pairs_checksum_device(ptr, n, ctypes.byref(s), ctypes.byref(x)))
     return s.value, x.value
+
+
+def graph_stats():
+    """(captures, replays) of launch sequences replayed as HIP graphs."""
+    c, r = U64(), U64()
+    check(lib().spj_graph_stats(ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
 
 
 def init(device=0):

@@ -1,0 +1,94 @@
+"""Repeated identical calls replay their launch sequence as a HIP graph.
+Every replay is checked against the eagerly launched (timed) call, with the
+program's own eager work between replays: the output buffer is overwritten
+with 0xFF bytes, checksums and copies run on other streams."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_and_check(sj, dR, nR, dS, nS, dO, cfg, want, reps, scratch):
+    for k in range(reps):
+        sj.check(sj.lib().spj_dev_memset(dO.ptr, 0xFF, nS * 8))   # a replay that wrote nothing would show
+        n, _ = sj.join_device(dR, nR, dS, nS, dO, cfg)
+        got = (n,) + sj.checksum_device(dO.ptr, n)
+        assert got == want, k
+        ms = ctypes.c_float()
+        sj.check(sj.lib().spj_d2d(scratch.ptr, dO.ptr, min(nS, 1 << 16) * 8, ctypes.byref(ms)))
+
+
+@pytest.mark.parametrize("nR,nS,zipf,alg", [(1 << 20, 1 << 22, 1.2, 0), (1 << 18, 1 << 20, 0.4, 2),
+                                             (40_000, 160_000, 0.7, -1), ((1 << 21) + 7, 1 << 23, 0.0, 0)])
+def test_graph_replays_match_eager_launches(gpu, orc, nR, nS, zipf, alg):
+    sj = gpu
+    R = sj.gen_r(nR, 61 + nR)
+    S = sj.gen_s(nS, R, zipf, 62 + nR)
+    S["key"][::101] += np.uint32(1 << 24)   # unmatched S tuples: holes that the host compacts
+    want_oracle = orc.join_checksum(R, S)
+    cfg = sj.config(algorithm=alg) if alg >= 0 else None
+    dR, dS = sj.DeviceArray.from_numpy(R), sj.DeviceArray.from_numpy(S)
+    dO, scratch = sj.DeviceArray(nS * 8), sj.DeviceArray(1 << 20)
+    try:
+        print(f"case graph nR={nR} nS={nS} zipf={zipf} alg={alg}", flush=True)
+        n, _ = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)   # eager
+        want = (n,) + sj.checksum_device(dO.ptr, n)
+        assert want == want_oracle
+        c0, r0 = sj.graph_stats()
+        run_and_check(sj, dR, nR, dS, nS, dO, cfg, want, 6, scratch)
+        c1, r1 = sj.graph_stats()
+        assert c1 == c0 + 1 and r1 - r0 == 6   # captured at the first repeat of the eager call, replayed by all
+        # another call in between drops the graph; the same call again is
+        # launched directly, then captured anew
+        dO2 = sj.DeviceArray(nS * 8)
+        n2, _ = sj.join_device(dR, nR, dS, nS, dO2, cfg)
+        assert (n2,) + sj.checksum_device(dO2.ptr, n2) == want
+        dO2.free()
+        run_and_check(sj, dR, nR, dS, nS, dO, cfg, want, 4, scratch)
+        c2, r2 = sj.graph_stats()
+        assert c2 == c1 + 1 and r2 - r1 == 3
+    finally:
+        for d in (dR, dS, dO, scratch):
+            d.free()
+
+
+def test_graph_replays_of_a_batch(gpu, orc):
+    """The batched call replays too; each instance stays exact."""
+    sj = gpu
+    rng = np.random.default_rng(5)
+    Rs, Ss = [], []
+    for i in range(48):
+        nR = int(2 ** rng.uniform(6, 19))
+        R = sj.gen_r(nR, 70 + i)
+        Rs.append(R)
+        Ss.append(sj.gen_s(nR * int(rng.choice([1, 2, 4, 8])), R, float(rng.uniform(0, 1.25)), 170 + i))
+    want = [orc.join_checksum(R, S) for R, S in zip(Rs, Ss)]
+    n = len(Rs)
+    dRs = [sj.DeviceArray.from_numpy(r) for r in Rs]
+    dSs = [sj.DeviceArray.from_numpy(s) for s in Ss]
+    dOs = [sj.DeviceArray(max(len(s), 1) * 8) for s in Ss]
+    P = ctypes.c_void_p
+    rp = (P * n)(*[d.ptr for d in dRs])
+    sp = (P * n)(*[d.ptr for d in dSs])
+    op = (P * n)(*[d.ptr for d in dOs])
+    nr = (ctypes.c_uint64 * n)(*[len(r) for r in Rs])
+    ns = (ctypes.c_uint64 * n)(*[len(s) for s in Ss])
+    cnt = (ctypes.c_uint64 * n)()
+    try:
+        c0, r0 = sj.graph_stats()
+        for k in range(5):
+            for d, s in zip(dOs, Ss):
+                sj.check(sj.lib().spj_dev_memset(d.ptr, 0xFF, max(len(s), 1) * 8))
+            cfg = sj.config(algorithm=0)
+            sj.check(sj.lib().spj_join_batch_device(n, ctypes.cast(rp, P), ctypes.cast(nr, P), ctypes.cast(sp, P),
+                                                    ctypes.cast(ns, P), ctypes.cast(op, P), ctypes.cast(cnt, P),
+                                                    ctypes.byref(cfg), None))
+            for i in range(n):
+                assert (int(cnt[i]),) + sj.checksum_device(int(op[i]), int(cnt[i])) == want[i], (k, i)
+        c1, r1 = sj.graph_stats()
+        assert c1 == c0 + 1 and r1 - r0 == 4   # the first call launches directly
+    finally:
+        for d in dRs + dSs + dOs:
+            d.free()
