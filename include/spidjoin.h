@@ -63,7 +63,7 @@ typedef struct {
     int32_t part_layout;      /* partitions of R and S: 0 counted (a histogram pass sizes them
                                  exactly), 1 cells (no histogram pass: fixed-capacity cells per
                                  scatter group and digit; the join is redone counted when a cell
-                                 overflows); AUTO = cells for a hash-join relation of >= 2^20 tuples
+                                 overflows); AUTO = cells for a hash or sort-merge relation of >= 2^20 tuples
                                  (for S only with heavy-key handling on) */
     int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
 } spj_config_t;

@@ -304,7 +304,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj;
         auto use_cells = [&](uint32_t len, bool skew_safe) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
-                   (c.cells == 1 || (c.cells == 2 && c.alg == 0 && len >= (1u << 20) && skew_safe));
+                   (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= (1u << 20) && skew_safe));
         };
         cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true);
         cells[1][i] = !nphj && use_cells(s.nS, heavy_on);

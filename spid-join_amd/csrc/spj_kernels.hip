@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "spj_common.hpp"
@@ -1078,6 +1079,7 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;   // at most 8192 buckets (16-b
 constexpr uint32_t kTableCap = 17400;       // 8 B per tuple + 16 KB of bucket ends
 constexpr uint32_t kSubFill = 16384;        // mean sub-table fill when R_p needs several tables
 constexpr uint32_t kResplits = 2;           // sub-table overflows re-split (2Q) before sort-merge
+constexpr uint32_t kRangeSortMax = 64;      // sort-merge: longest key-range bucket sorted in place
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
 constexpr uint32_t kSortTail = 1024;        // sort-merge: a second, short run in the same pass
@@ -1092,6 +1094,9 @@ struct JoinShared {
     uint32_t fill;   // pairs written in the item's reservation
     uint32_t base;   // reservation start
     uint32_t res;    // reservation length
+    uint32_t kmin;   // sort-merge: the smallest and largest R key of the partition
+    uint32_t kmax;
+    uint32_t big;    // sort-merge: some key-range bucket is too long to sort in place
 };
 
 // Append one pair per lane whose `hit` is set.  Positions come from a
@@ -1269,14 +1274,18 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
         uint32_t alg = I.alg;
         if (alg == 0 && nr <= kNLMax) alg = 3;
         uint32_t *erri = err + w.inst;
-        // set (uniformly) once the item has taken the next item from the
-        // queue and its output reservation: the hash path does both at its
-        // first sub-table, the other paths at their start and end
+        // The table paths (hash, and sort-merge of a partition that fits one
+        // table) take the next item from the queue and the item's output
+        // reservation at their first table, the other paths at their start
+        // (reservation) and end (queue).  `advanced` is set (uniformly) once
+        // both are taken; exactly one reservation is made per item.
+        const bool table_path = alg == 0 || (alg == 2 && nr <= kTableCap);
         bool advanced = false;
+        bool run_sort = false;   // sort-merge by sorted runs, not key-range buckets
         if (threadIdx.x == 0) {
             sh.res = w.se - w.sb;
             sh.fill = 0;
-            if (alg != 0) sh.base = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+            if (!table_path) sh.base = atomicAdd(&out_lo[w.inst], w.se - w.sb);
         }
         __syncthreads();
 
@@ -1306,8 +1315,10 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         }
                     }
                 }
-            } else if (alg == 2) {
-                // ---------------- sort-merge (mpsm_join_sort_probe.c): the R
+            } else if (alg == 2 && (run_sort || nr > kTableCap)) {
+                // ---------------- sort-merge in runs (mpsm_join_sort_probe.c),
+                // for a partition larger than one table or with keys too
+                // clustered for the key-range buckets below: the R
                 // partition is bitonic-sorted by key in LDS (runs of kSortR);
                 // each S tuple then takes the R entry at the lower bound of its
                 // key, found by binary search -- the entry the reference's merge
@@ -1433,268 +1444,413 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     c0 += cn + cn2;
                 }
             } else {
-                // ---------------- hash join
-                // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
-                // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
-                // bucket b is [bend[b], bend[b + 1]).
-                // During the count pass the 32-bit bucket counters live in the
-                // pair area, which the fill pass overwrites afterwards.
-                uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
-                uint32_t *bcnt = lds;                           // nb counters (count pass only)
-                uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
-                uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
-                // A partition larger than one table is split by high hash bits
-                // into Q sub-tables of kSubFill tuples on average (the slack up
-                // to kTableCap covers the spread of their sizes).  One that still
-                // overflows splits the partition again, into 2Q sub-tables, from
-                // the start; after kResplits such splits it goes to sort-merge.
-                uint32_t Q = nr <= kTableCap ? 1u : (nr + kSubFill - 1) / kSubFill;
-                uint32_t splits = 0;
-                bool to_smj = false;
-                const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
-                constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
-                for (uint32_t q = 0; q < Q;) {
-                    __syncthreads();
-                    {
-                        uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
-                        asm volatile("" : "+v"(i0));
-                        for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
-                    }
-                    __syncthreads();
-                    // R_p fits one table (Q == 1): hold it in registers (<= RPT
-                    // tuples per thread) and read it once; otherwise each
-                    // sub-table reads R_p twice, keeping its own keys
-                    uint2 r[RPT];
-#pragma unroll
-                    for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
-                    if (Q == 1) {
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) {
-                            const uint32_t i = rb + u * NT + threadIdx.x;
-                            if (i < re) r[u] = gld(I.Rp, rpos(i));
+                // ---------------- hash join (and sort-merge of a partition
+                // that fits one table: RANGE, key-range buckets)
+                bool to_smj = false;   // the item goes on to the run sort
+                auto table_join = [&](auto range_tag) {
+                    constexpr bool RANGE = decltype(range_tag)::value;
+
+                    // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
+                    // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
+                    // bucket b is [bend[b], bend[b + 1]).
+                    // During the count pass the 32-bit bucket counters live in the
+                    // pair area, which the fill pass overwrites afterwards.
+                    uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
+                    uint32_t *bcnt = lds;                           // nb counters (count pass only)
+                    uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
+                    uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
+                    // A partition larger than one table is split by high hash bits
+                    // into Q sub-tables of kSubFill tuples on average (the slack up
+                    // to kTableCap covers the spread of their sizes).  One that still
+                    // overflows splits the partition again, into 2Q sub-tables, from
+                    // the start; after kResplits such splits it goes to sort-merge.
+                    uint32_t Q = nr <= kTableCap ? 1u : (nr + kSubFill - 1) / kSubFill;
+                    uint32_t splits = 0;
+                    const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
+                    constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
+                    // the bucket of a key: multiply-shift hashing, or (sort-merge)
+                    // its place in the partition's key range [kmin, kmax] cut into
+                    // nb equal ranges -- monotone in the key, so the buckets in
+                    // order, each sorted, are the partition sorted by key
+                    uint32_t rmin = 0, rscale = 0;
+                    auto bkt = [&](uint32_t k) -> uint32_t {
+                        if constexpr (RANGE) {
+                            const uint32_t g = (uint32_t)(((uint64_t)(k - rmin) * rscale) >> 32);
+                            return g < nb ? g : nb - 1u;   // a key outside the range: any bucket (it matches no entry)
+                        } else {
+                            return bucket_of(k, lb);
                         }
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++)
-                            if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bucket_of(r[u].x, lb)], 1u);
-                    } else {
-                        for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                            uint2 c[4];
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const uint32_t i = i0 + u * NT + threadIdx.x;
-                                c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                                atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
-                            }
+                    };
+                    for (uint32_t q = 0; q < Q;) {
+                        __syncthreads();
+                        {
+                            uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
+                            asm volatile("" : "+v"(i0));
+                            for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
                         }
-                    }
-                    __syncthreads();
-                    const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
-                    if (ntab > kTableCap) {   // uniform: ntab is the same in every thread
-                        if (!advanced) {   // the queue and reservation atomics, as below
-                            uint32_t nx = 0, rbx = 0;
-                            if (threadIdx.x == 0) {
-                                nx = atomicAdd(&ctrl[1], 1u);
-                                rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
-                            }
-                            __builtin_amdgcn_s_waitcnt(0);
-                            if (threadIdx.x == 0) {
-                                sh.item = sh.item2;
-                                sh.item2 = nx;
-                                sh.base = rbx;
-                            }
-                            advanced = true;
-                        }
-                        // the earlier sub-tables' pairs are written again, from
-                        // the reservation's start (every barrier since their
-                        // probe has passed, so no wave still appends)
-                        if (threadIdx.x == 0) sh.fill = 0;
-                        if (splits == kResplits) {
-                            to_smj = true;
-                            break;
-                        }
-                        splits++;
-                        Q *= 2;
-                        q = 0;
-                        continue;
-                    }
-                    // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
-                    // cursor of bucket b is bend[b + 1]
-                    {
-                        // the lane index is made opaque so that the addresses are
-                        // formed here: hoisted out of the item loop they spill,
-                        // and the reload waits (in order) for the previous item's
-                        // pair stores
-                        uint32_t i0 = threadIdx.x;
-                        asm volatile("" : "+v"(i0));
-                        for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
-                    }
-                    if (threadIdx.x == 0) bend[0] = 0;
-                    __syncthreads();
-                    // fill pass: a packed atomic advances one 16-bit cursor of a
-                    // word; it never carries, every cursor staying <= ntab
-                    if (Q == 1) {
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) {
-                            if (r[u].x == kEmptyKey) continue;
-                            const uint32_t c = bucket_of(r[u].x, lb) + 1u, sh16 = (c & 1u) * 16u;
-                            const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
-                            tkv[pos] = r[u];
-                        }
-                    } else {
-                        for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
-                            uint2 c[4];
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const uint32_t i = i0 + u * NT + threadIdx.x;
-                                c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                                const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
-                                const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
-                                tkv[pos] = c[u];
-                            }
-                        }
-                    }
-                    if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
-                    __syncthreads();   // bend[b + 1] = end of bucket b
-                    // duplicate R keys (a2a_srj_build.c:265-271): each entry against
-                    // the later entries of its bucket, in phases (keys, bucket ends,
-                    // next entry) so that the LDS reads of all entries overlap;
-                    // buckets of more than two entries finish in a short loop
-                    {
-                        bool dup = false;
-                        uint32_t kk[RPT], ee[RPT];
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) {
-                            const uint32_t i = u * NT + threadIdx.x;
-                            kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
-                        }
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
-                        // the next kScan entries are read unconditionally (the
-                        // table has kScan slack slots past its last entry), so
-                        // that the reads overlap; only buckets longer than that
-                        // finish in a loop
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) {
-                            const uint32_t i = u * NT + threadIdx.x;
-                            if (i >= ntab) continue;
-#pragma unroll
-                            for (uint32_t d = 1; d <= kScan; d++) {
-                                const uint32_t kd = tkv[i + d].x;   // i + d < ntab + kScan: inside the table
-                                dup |= (i + d < ee[u]) & (kd == kk[u]);
-                            }
-                        }
-#pragma unroll
-                        for (uint32_t u = 0; u < RPT; u++) {
-                            const uint32_t i = u * NT + threadIdx.x;
-                            for (uint32_t j = i + kScan + 1; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
-                        }
-                        if (dup) atomicOr(erri, kErrDuplicateKey);
-                    }
-                    // probe: the S range is cut into chunks of a wave's 64 x IPT
-                    // tuples; wave i starts on chunk i and then takes chunks from
-                    // a counter, so that waves that finish early take over the
-                    // rest instead of waiting at the item's end for slow ones.
-                    // The next chunk's loads are in flight while one is probed.
-                    constexpr uint32_t CH = kWave * IPT;
-                    const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
-                    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
-                    const uint32_t ln = lane_id();
-                    uint2 nv[IPT];
-                    {
-                        // the lane index is made opaque so that the first chunk's
-                        // positions are mapped here, not hoisted to the item's
-                        // start (where they would spill, and their reload wait,
-                        // in order, behind every store in flight)
-                        uint32_t lane = ln;
-                        asm volatile("" : "+v"(lane));
-                        load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
-                    }
-                    uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
-                    if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
-                    uint32_t nx = 0, rbx = 0, pre = 0;
-                    // the next item's words: wave 0, lane i loads word i
-                    const uint32_t it2 = !advanced && threadIdx.x < kItemWords ? sh.item2 : total_items;
-                    if (it2 < total_items) {
-                        // the lane offset is made opaque so that the address is
-                        // formed here, not hoisted out of the item loop (a
-                        // 64-bit per-lane pointer there would spill)
-                        uint32_t lane = threadIdx.x;
-                        asm volatile("" : "+v"(lane));
-                        pre = gld32(reinterpret_cast<const uint32_t *>(items + it2), lane);
-                    }
-                    if (!advanced && threadIdx.x == 0) {
-                        nx = atomicAdd(&ctrl[1], 1u);
-                        rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
-                    }
-                    __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
-                    if (!advanced) {
-                        if (it2 < total_items) nw[threadIdx.x] = pre;
-                        if (threadIdx.x == 0) {
-                            if (it2 < total_items) nw_item = it2;
-                            sh.item = sh.item2;   // read at the next item's start
-                            sh.item2 = nx;
-                            sh.base = rbx;        // read by the pair emission
+                        if (RANGE && threadIdx.x == 0) {
+                            sh.kmin = 0xFFFFFFFFu;
+                            sh.kmax = 0u;
+                            sh.big = 0u;
                         }
                         __syncthreads();
-                        advanced = true;
-                    }
-                    uint32_t cn = wv < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;   // this wave's next chunk
-                    for (uint32_t c = wv; c < nch;) {
-                        uint2 v[IPT];
+                        // R_p fits one table (Q == 1): hold it in registers (<= RPT
+                        // tuples per thread) and read it once; otherwise each
+                        // sub-table reads R_p twice, keeping its own keys
+                        uint2 r[RPT];
 #pragma unroll
-                        for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
-                        if (cn < nch) load_chunk<IPT>(nv, I.Sp, spos, w.sb + cn * CH, w.se, ln);
-                        cg = 0;
-                        if (ln == 0 && cn < nch) cg = atomicAdd(&sh.chunk, 1u);   // the chunk after that
-                        uint32_t hitm = 0;
+                        for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
+                        if (Q == 1) {
 #pragma unroll
-                        for (int j = 0; j < (int)IPT; j++) {
-                            // branch-free: key 0 and padding tuples simply find
-                            // no entry (the table holds no key 0), nor does a key
-                            // of another sub-table of a split partition; table
-                            // keys are unique (duplicates are an error), so at
-                            // most one entry matches
-                            const uint32_t k = v[j].x;
-                            const uint32_t bk = bucket_of(k, lb);
-                            const uint32_t st = bend[bk], n = bend[bk + 1] - st;
-                            bool hit = false;
-                            uint32_t rr = 0;
-#pragma unroll
-                            for (uint32_t u = 0; u < kScan; u++) {
-                                const uint2 e = tkv[st + u];   // the table has kScan padding slots
-                                const bool m = (u < n) && (e.x == k);
-                                rr = m ? e.y : rr;
-                                hit = hit || m;
+                            for (uint32_t u = 0; u < RPT; u++) {
+                                const uint32_t i = rb + u * NT + threadIdx.x;
+                                if (i < re) r[u] = gld(I.Rp, rpos(i));
                             }
-                            if (n > kScan && !hit) {   // a long bucket (rare)
-                                for (uint32_t i = st + kScan; i < st + n; i++) {
-                                    const uint2 e = tkv[i];
-                                    if (e.x == k) { rr = e.y; hit = true; break; }
+                            if constexpr (RANGE) {
+                                // the key range: per-thread, per-wave, then one LDS
+                                // atomic per wave (sh.kmin / kmax were set at the clear)
+                                uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++)
+                                    if (r[u].x != kEmptyKey) {
+                                        lo = min(lo, r[u].x);
+                                        hi = max(hi, r[u].x);
+                                    }
+#pragma unroll
+                                for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+                                    lo = min(lo, (uint32_t)__shfl_xor((int)lo, (int)off, kWave));
+                                    hi = max(hi, (uint32_t)__shfl_xor((int)hi, (int)off, kWave));
+                                }
+                                if (lane_id() == 0) {
+                                    atomicMin(&sh.kmin, lo);
+                                    atomicMax(&sh.kmax, hi);
+                                }
+                                __syncthreads();
+                                rmin = sh.kmin;
+                                const uint32_t rmax = sh.kmax;
+                                // scale = floor(nb 2^32 / (range)), so a key's bucket
+                                // (k - kmin) scale >> 32 is below nb inside the range
+                                rscale = rmin <= rmax ? (uint32_t)min<uint64_t>(((uint64_t)nb << 32) / ((uint64_t)(rmax - rmin) + 1u),
+                                                                                0xFFFFFFFFull)
+                                                      : 0u;
+                            }
+#pragma unroll
+                            for (uint32_t u = 0; u < RPT; u++)
+                                if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bkt(r[u].x)], 1u);
+                        } else {
+                            for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                                uint2 c[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const uint32_t i = i0 + u * NT + threadIdx.x;
+                                    c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                    atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
                                 }
                             }
-                            v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
-                            hitm |= (hit ? 1u : 0u) << j;
                         }
-                        // wait here for the next chunk's loads (issued a probe
-                        // ago), not at the next chunk's start, where the wait
-                        // would also cover the stores below
-                        __builtin_amdgcn_s_waitcnt(0);
-                        c = cn;
-                        cn = cn < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;
-                        wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
+                        __syncthreads();
+                        const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                        if (ntab > kTableCap) {   // uniform: ntab is the same in every thread
+                            if (!advanced) {   // the queue and reservation atomics, as below
+                                uint32_t nx = 0, rbx = 0;
+                                if (threadIdx.x == 0) {
+                                    nx = atomicAdd(&ctrl[1], 1u);
+                                    rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                                }
+                                __builtin_amdgcn_s_waitcnt(0);
+                                if (threadIdx.x == 0) {
+                                    sh.item = sh.item2;
+                                    sh.item2 = nx;
+                                    sh.base = rbx;
+                                }
+                                advanced = true;
+                            }
+                            // the earlier sub-tables' pairs are written again, from
+                            // the reservation's start (every barrier since their
+                            // probe has passed, so no wave still appends)
+                            if (threadIdx.x == 0) sh.fill = 0;
+                            if (splits == kResplits) {
+                                to_smj = true;
+                                break;
+                            }
+                            splits++;
+                            Q *= 2;
+                            q = 0;
+                            continue;
+                        }
+                        // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
+                        // cursor of bucket b is bend[b + 1]
+                        {
+                            // the lane index is made opaque so that the addresses are
+                            // formed here: hoisted out of the item loop they spill,
+                            // and the reload waits (in order) for the previous item's
+                            // pair stores
+                            uint32_t i0 = threadIdx.x;
+                            asm volatile("" : "+v"(i0));
+                            for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                        }
+                        if (threadIdx.x == 0) bend[0] = 0;
+                        __syncthreads();
+                        // fill pass: a packed atomic advances one 16-bit cursor of a
+                        // word; it never carries, every cursor staying <= ntab
+                        if (Q == 1) {
+#pragma unroll
+                            for (uint32_t u = 0; u < RPT; u++) {
+                                if (r[u].x == kEmptyKey) continue;
+                                const uint32_t c = bkt(r[u].x) + 1u, sh16 = (c & 1u) * 16u;
+                                const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                                tkv[pos] = r[u];
+                            }
+                        } else {
+                            for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
+                                uint2 c[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const uint32_t i = i0 + u * NT + threadIdx.x;
+                                    c[u] = (i < re) ? gld(I.Rp, rpos(i)) : make_uint2(kEmptyKey, 0u);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                    const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
+                                    const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                                    tkv[pos] = c[u];
+                                }
+                            }
+                        }
+                        if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
+                        __syncthreads();   // bend[b + 1] = end of bucket b
+                        if constexpr (RANGE) {
+                            // sort each key-range bucket in place (its entries are
+                            // contiguous): four or fewer in registers by a network,
+                            // up to kRangeSortMax by insertion; a longer one (keys
+                            // clustered in the range) sends the item to the run
+                            // sort below.  Equal keys share a bucket and end up
+                            // adjacent: duplicates (mpsm_join_sort_probe.c sorts each
+                            // range bucket, 152-236; a2a_srj_build.c:265-271).
+                            bool dup = false, big = false;
+                            uint32_t b0 = threadIdx.x;   // opaque: addresses formed here
+                            asm volatile("" : "+v"(b0));
+                            for (uint32_t bb = b0; bb < nb; bb += NT) {
+                                const uint32_t st = bend[bb], n = bend[bb + 1] - st;
+                                if (n <= 1) continue;
+                                if (n <= 4) {
+                                    // (key, position) as 64-bit words, so that the
+                                    // order is strict; slots past n sort last
+                                    uint64_t e[4];
+                                    uint32_t rid[4];
+#pragma unroll
+                                    for (uint32_t u = 0; u < 4; u++) {
+                                        const uint2 t = tkv[st + u];   // the table has kScan slack slots
+                                        e[u] = u < n ? ((uint64_t)t.x << 3) | u : ~0ull;
+                                        rid[u] = t.y;
+                                    }
+                                    auto cx = [&](int a, int c) {
+                                        const uint64_t x = e[a], y = e[c];
+                                        e[a] = x < y ? x : y;
+                                        e[c] = x < y ? y : x;
+                                    };
+                                    cx(0, 1);
+                                    cx(2, 3);
+                                    cx(0, 2);
+                                    cx(1, 3);
+                                    cx(1, 2);
+                                    uint2 o[4];
+#pragma unroll
+                                    for (uint32_t u = 0; u < 4; u++) {
+                                        const uint32_t src = (uint32_t)e[u] & 7u;
+                                        uint32_t rr = rid[0];
+                                        rr = src == 1u ? rid[1] : rr;
+                                        rr = src == 2u ? rid[2] : rr;
+                                        rr = src == 3u ? rid[3] : rr;
+                                        o[u] = make_uint2((uint32_t)(e[u] >> 3), rr);
+                                    }
+#pragma unroll
+                                    for (uint32_t u = 0; u < 4; u++)
+                                        if (u < n) tkv[st + u] = o[u];
+#pragma unroll
+                                    for (uint32_t u = 1; u < 4; u++) dup |= u < n && o[u].x == o[u - 1].x;
+                                } else if (n <= kRangeSortMax) {
+                                    for (uint32_t a = st + 1; a < st + n; a++) {   // insertion sort (rare)
+                                        const uint2 t = tkv[a];
+                                        uint32_t c = a;
+                                        while (c > st && tkv[c - 1].x > t.x) {
+                                            tkv[c] = tkv[c - 1];
+                                            c--;
+                                        }
+                                        tkv[c] = t;
+                                        dup |= c > st && tkv[c - 1].x == t.x;
+                                    }
+                                } else {
+                                    big = true;
+                                }
+                            }
+                            if (dup) atomicOr(erri, kErrDuplicateKey);
+                            if (big) sh.big = 1u;
+                            __syncthreads();   // the buckets are sorted; sh.big is final
+                            if (sh.big) {
+                                // too long a bucket: the run sort takes the item
+                                // (once the queue and reservation atomics are done,
+                                // as an overflowing hash table does)
+                                if (!advanced) {
+                                    uint32_t nx = 0, rbx = 0;
+                                    if (threadIdx.x == 0) {
+                                        nx = atomicAdd(&ctrl[1], 1u);
+                                        rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                                    }
+                                    __builtin_amdgcn_s_waitcnt(0);
+                                    if (threadIdx.x == 0) {
+                                        sh.item = sh.item2;
+                                        sh.item2 = nx;
+                                        sh.base = rbx;
+                                    }
+                                    advanced = true;
+                                }
+                                to_smj = true;
+                                break;
+                            }
+                        } else {
+                            // duplicate R keys (a2a_srj_build.c:265-271): each entry against
+                            // the later entries of its bucket, in phases (keys, bucket ends,
+                            // next entry) so that the LDS reads of all entries overlap;
+                            // buckets of more than two entries finish in a short loop
+                            {
+                                bool dup = false;
+                                uint32_t kk[RPT], ee[RPT];
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++) {
+                                    const uint32_t i = u * NT + threadIdx.x;
+                                    kk[u] = (i < ntab) ? tkv[i].x : kEmptyKey;
+                                }
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++) ee[u] = (kk[u] != kEmptyKey) ? bend[bucket_of(kk[u], lb) + 1] : 0u;
+                                // the next kScan entries are read unconditionally (the
+                                // table has kScan slack slots past its last entry), so
+                                // that the reads overlap; only buckets longer than that
+                                // finish in a loop
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++) {
+                                    const uint32_t i = u * NT + threadIdx.x;
+                                    if (i >= ntab) continue;
+#pragma unroll
+                                    for (uint32_t d = 1; d <= kScan; d++) {
+                                        const uint32_t kd = tkv[i + d].x;   // i + d < ntab + kScan: inside the table
+                                        dup |= (i + d < ee[u]) & (kd == kk[u]);
+                                    }
+                                }
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++) {
+                                    const uint32_t i = u * NT + threadIdx.x;
+                                    for (uint32_t j = i + kScan + 1; j < ee[u]; j++) dup |= (tkv[j].x == kk[u]);
+                                }
+                                if (dup) atomicOr(erri, kErrDuplicateKey);
+                            }
+                        }
+                        // probe: the S range is cut into chunks of a wave's 64 x IPT
+                        // tuples; wave i starts on chunk i and then takes chunks from
+                        // a counter, so that waves that finish early take over the
+                        // rest instead of waiting at the item's end for slow ones.
+                        // The next chunk's loads are in flight while one is probed.
+                        constexpr uint32_t CH = kWave * IPT;
+                        const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
+                        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+                        const uint32_t ln = lane_id();
+                        uint2 nv[IPT];
+                        {
+                            // the lane index is made opaque so that the first chunk's
+                            // positions are mapped here, not hoisted to the item's
+                            // start (where they would spill, and their reload wait,
+                            // in order, behind every store in flight)
+                            uint32_t lane = ln;
+                            asm volatile("" : "+v"(lane));
+                            load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
+                        }
+                        uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
+                        if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
+                        uint32_t nx = 0, rbx = 0, pre = 0;
+                        // the next item's words: wave 0, lane i loads word i
+                        const uint32_t it2 = !advanced && threadIdx.x < kItemWords ? sh.item2 : total_items;
+                        if (it2 < total_items) {
+                            // the lane offset is made opaque so that the address is
+                            // formed here, not hoisted out of the item loop (a
+                            // 64-bit per-lane pointer there would spill)
+                            uint32_t lane = threadIdx.x;
+                            asm volatile("" : "+v"(lane));
+                            pre = gld32(reinterpret_cast<const uint32_t *>(items + it2), lane);
+                        }
+                        if (!advanced && threadIdx.x == 0) {
+                            nx = atomicAdd(&ctrl[1], 1u);
+                            rbx = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                        }
+                        __builtin_amdgcn_s_waitcnt(0);   // enter with no loads in flight (see the wait below)
+                        if (!advanced) {
+                            if (it2 < total_items) nw[threadIdx.x] = pre;
+                            if (threadIdx.x == 0) {
+                                if (it2 < total_items) nw_item = it2;
+                                sh.item = sh.item2;   // read at the next item's start
+                                sh.item2 = nx;
+                                sh.base = rbx;        // read by the pair emission
+                            }
+                            __syncthreads();
+                            advanced = true;
+                        }
+                        uint32_t cn = wv < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;   // this wave's next chunk
+                        for (uint32_t c = wv; c < nch;) {
+                            uint2 v[IPT];
+#pragma unroll
+                            for (int j = 0; j < (int)IPT; j++) v[j] = nv[j];
+                            if (cn < nch) load_chunk<IPT>(nv, I.Sp, spos, w.sb + cn * CH, w.se, ln);
+                            cg = 0;
+                            if (ln == 0 && cn < nch) cg = atomicAdd(&sh.chunk, 1u);   // the chunk after that
+                            uint32_t hitm = 0;
+#pragma unroll
+                            for (int j = 0; j < (int)IPT; j++) {
+                                // branch-free: key 0 and padding tuples simply find
+                                // no entry (the table holds no key 0), nor does a key
+                                // of another sub-table of a split partition; table
+                                // keys are unique (duplicates are an error), so at
+                                // most one entry matches
+                                const uint32_t k = v[j].x;
+                                const uint32_t bk = bkt(k);
+                                const uint32_t st = bend[bk], n = bend[bk + 1] - st;
+                                bool hit = false;
+                                uint32_t rr = 0;
+#pragma unroll
+                                for (uint32_t u = 0; u < kScan; u++) {
+                                    const uint2 e = tkv[st + u];   // the table has kScan padding slots
+                                    const bool m = (u < n) && (e.x == k);
+                                    rr = m ? e.y : rr;
+                                    hit = hit || m;
+                                }
+                                if (n > kScan && !hit) {   // a long bucket (rare)
+                                    for (uint32_t i = st + kScan; i < st + n; i++) {
+                                        const uint2 e = tkv[i];
+                                        if (e.x == k) { rr = e.y; hit = true; break; }
+                                    }
+                                }
+                                v[j].x = rr;   // the pair (R rid, S rid), kept only if hit
+                                hitm |= (hit ? 1u : 0u) << j;
+                            }
+                            // wait here for the next chunk's loads (issued a probe
+                            // ago), not at the next chunk's start, where the wait
+                            // would also cover the stores below
+                            __builtin_amdgcn_s_waitcnt(0);
+                            c = cn;
+                            cn = cn < nch ? __builtin_amdgcn_readfirstlane(cg) : nch;
+                            wave_emit_round<IPT>(hitm, v, &sh, I.out, erri);
+                        }
+                        q++;
                     }
-                    q++;
-                }
-                if (to_smj) {   // sh.fill was reset above
+                };
+                if (alg == 2) table_join(std::true_type{});
+                else table_join(std::false_type{});
+                if (to_smj) {   // sh.fill was reset above (hash), or nothing was written (sort-merge)
                     alg = 2;
+                    run_sort = true;
                     continue;
                 }
             }

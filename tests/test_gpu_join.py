@@ -625,3 +625,44 @@ def test_oversized_partition_resplit_and_sort_merge_fallback(gpu, orc, layout):
         with pytest.raises(sj.SpjError) as e:
             sj.join(R2, S, cfg)
         assert e.value.code == -4
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_sort_merge_key_range_buckets(gpu, orc, layout):
+    """Sort-merge of partitions that fit one table: R is counting-sorted into
+    key-range buckets and each bucket sorted in place.  Keys at the ends of
+    the uint32 range (1 and 0xFFFFFFFF), S keys below and above R's range,
+    a partition whose keys crowd a few buckets (the items go to the run
+    sort), and a duplicate inside one range bucket."""
+    sj = gpu
+    rng = np.random.default_rng(321)
+    n = 1 << 18
+    keys = rng.permutation(np.unique(rng.integers(2, 1 << 31, 2 * n, dtype=np.uint64)))[:n].astype(np.uint32)
+    keys[:2] = [1, 0xFFFFFFFF]
+    R = np.zeros(n, dtype=sj.TUPLE)
+    R["key"] = keys
+    R["rid"] = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    S = np.zeros(4 * n, dtype=sj.TUPLE)
+    S["key"] = keys[rng.integers(0, n, len(S))]
+    S["key"][::50] = rng.integers(1 << 31, 1 << 32, len(S[::50]), dtype=np.uint64).astype(np.uint32)   # mostly absent
+    S["rid"] = np.arange(len(S), dtype=np.uint32)
+    cfg = sj.config(algorithm=2, part_layout=layout)
+    print(f"case smj range layout={layout}", flush=True)
+    pairs, _ = sj.join(R, S, cfg)
+    check_equal(sj, orc, R, S, pairs)
+    # clustered keys: a dense run of consecutive keys next to a few far ones
+    # puts thousands of keys in one range bucket of each partition
+    R2 = R.copy()
+    R2["key"][: n // 2] = np.arange(1000, 1000 + n // 2, dtype=np.uint32)
+    far = rng.permutation(np.unique(rng.integers(1 << 30, 1 << 31, 2 * n, dtype=np.uint64)))[: n - n // 2]
+    R2["key"][n // 2:] = far.astype(np.uint32)
+    S2 = S.copy()
+    S2["key"] = R2["key"][rng.integers(0, n, len(S2))]
+    pairs, _ = sj.join(R2, S2, cfg)
+    check_equal(sj, orc, R2, S2, pairs)
+    # a duplicate: the same key twice lands in the same range bucket
+    for R3 in (R.copy(), R2.copy()):
+        R3["key"][7] = R3["key"][n - 3]
+        with pytest.raises(sj.SpjError) as e:
+            sj.join(R3, S, cfg)
+        assert e.value.code == -4
