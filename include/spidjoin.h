@@ -47,8 +47,10 @@ enum { SPJ_ALG_AUTO = -1, SPJ_ALG_PHJ = 0, SPJ_ALG_NPHJ = 1, SPJ_ALG_SMJ = 2, SP
 
 /* Tunables; -1 = automatic (tuner table, else built-in defaults). */
 typedef struct {
-    int32_t algorithm;        /* SPJ_ALG_*: AUTO = NPHJ when the call's R relations total
-                                 <= 2^20 tuples and no partition parameter below is set,
+    int32_t algorithm;        /* SPJ_ALG_*: AUTO = NPHJ when no partition parameter below is
+                                 set, no tuner entry covers the instance, and the call is
+                                 small: one join with |R| <= 2^21 and |R| |S| <= 2^44, or a
+                                 batch whose totals have |R| <= 2^20 and |R| |S| <= 2^44;
                                  else hash, nested loop for tiny partitions;
                                  NPHJ = no partition pass: one open-addressing table of
                                  >= 2|R| 8-byte slots in device memory, probed by every S tuple */

@@ -63,26 +63,55 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
     return r;
 }
 
-// AUTO picks the no-partition join for a call whose R relations total at
-// most 2^20 tuples, unless a partition parameter is set.  Its tables (16 B
-// per R tuple) then stay in the caches and there is no partition pass: on
-// MI355X it took 0.07-0.19 ms where the partitioned join took 0.09-0.35 ms
-// (|R| = 2^12..2^20, |S| = 4|R|), and it lost from |R| = 2^22 on.  A batch
-// is judged by its total, since the tables of a large batch do not stay
-// cached.
-constexpr uint64_t kAutoNphjMaxR = uint64_t(1) << 20;
+// AUTO's choice of the no-partition join, measured on MI355X (device
+// relations, Zipf 0.5, ms per call, partitioned / no-partition):
+//   |R|     S:R = 4          S:R = 64         S:R = 256
+//   2^18    0.301 / 0.090    0.460 / 0.238    0.804 / 0.783
+//   2^19    0.260 / 0.130    0.577 / 0.515    1.470 / 1.724
+//   2^20    0.301 / 0.200    0.927 / 1.152    2.662 / 4.163
+//   2^21    0.369 / 0.310    1.554 / 2.507
+//   2^22    0.365 / 0.645    2.695 / 5.810
+// Without a partition pass the join wins while its table (16 B per R
+// tuple) stays cached and the partition pipeline's fixed cost dominates;
+// each S tuple's random table read costs more the larger R is.  A single
+// call takes it for |R| <= 2^21 and |R| |S| <= 2^44.  A batch is judged by
+// its totals, with |R| <= 2^20: the tables of a large batch do not stay
+// cached (1024 mixed instances: 6.7 ms partitioned, 16.0 ms without).
+constexpr uint64_t kAutoNphjMaxR = uint64_t(1) << 21;
+constexpr uint64_t kAutoNphjMaxBatchR = uint64_t(1) << 20;
+constexpr uint32_t kAutoNphjMaxLog2RS = 44;
+constexpr uint64_t kNphjMaxR = uint64_t(1) << 30;   // its table's slot index is 32-bit
 
-bool auto_no_partition(const spj_config_t *cfg, uint64_t total_r)
+static bool auto_sizes(uint64_t r, uint64_t s, uint64_t max_r)
 {
-    if (total_r > kAutoNphjMaxR) return false;
+    if (r > max_r) return false;
+    // |R| |S| <= 2^44 without overflow: |S| <= 2^44 / |R|
+    return r == 0 || s <= (uint64_t(1) << kAutoNphjMaxLog2RS) / r;
+}
+
+static bool auto_no_partition(const spj_config_t *cfg, uint64_t r, uint64_t s, uint64_t max_r)
+{
+    if (!auto_sizes(r, s, max_r)) return false;
     if (!cfg) return true;
     return cfg->algorithm < 0 && cfg->radix_bits < 0 && cfg->table_log2 < 0 && cfg->heavy < 0 &&
            cfg->heavy_min_count < 0 && cfg->scatter_ipt < 0 && cfg->tile_tuples <= 0 && cfg->items_s_max <= 0 &&
            cfg->part_layout < 0 && cfg->cell_capacity_pct < 0;
 }
 
-int check_resolved(const Resolved &r)
+// AUTO's no-partition choice for one instance: the rule above, unless the
+// tuner holds an entry for the instance's bucket, which then decides (its
+// entry may itself be the no-partition join).
+bool auto_no_partition_single(const spj_config_t *cfg, uint64_t nR, uint64_t nS)
 {
+    return auto_no_partition(cfg, nR, nS, kAutoNphjMaxR) && !tuner_has_entry((uint32_t)nR, (uint32_t)nS, -1.0);
+}
+
+int check_resolved(const Resolved &r, uint64_t nR)
+{
+    if (r.alg == SPJ_ALG_NPHJ && nR > kNphjMaxR)
+        return fail(SPJ_EINVAL, "the no-partition join takes at most 2^30 R tuples (its table has 2^k >= 2|R| "
+                                "slots), got %llu",
+                    (unsigned long long)nR);
     if (r.bits > kMaxBits) return fail(SPJ_EINVAL, "radix_bits %u > %u", r.bits, kMaxBits);
     if (r.table_log2 < 6 || r.table_log2 > 13) return fail(SPJ_EINVAL, "table_log2 %u outside 6..13", r.table_log2);
     if (r.alg > 3) return fail(SPJ_EINVAL, "algorithm %u", r.alg);
@@ -272,8 +301,8 @@ int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uin
     s.nR = (uint32_t)nR;
     s.nS = (uint32_t)nS;
     s.cfg = resolve(cfg, nR, nS, -1.0);
-    if (auto_no_partition(cfg, nR)) s.cfg.alg = SPJ_ALG_NPHJ;
-    rc = check_resolved(s.cfg);
+    if (auto_no_partition_single(cfg, nR, nS)) s.cfg.alg = SPJ_ALG_NPHJ;
+    rc = check_resolved(s.cfg, nR);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(default_context_mutex());
     std::vector<uint32_t> counts, errs;
@@ -379,9 +408,14 @@ int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR
 {
     if (n < 0 || (n > 0 && (!R || !nR || !S || !nS || !out || !out_count))) return fail(SPJ_EINVAL, "null argument");
     std::vector<InstSpec> specs((size_t)n);
-    uint64_t total_r = 0;
-    for (int i = 0; i < n; i++) total_r += std::min<uint64_t>(nR[i], kAutoNphjMaxR + 1);   // no wrap-around
-    const bool nphj = auto_no_partition(cfg, total_r);
+    uint64_t total_r = 0, total_s = 0;
+    for (int i = 0; i < n; i++) {   // saturating: no wrap-around
+        total_r = std::min(total_r + std::min<uint64_t>(nR[i], kAutoNphjMaxBatchR + 1), kAutoNphjMaxBatchR + 1);
+        total_s = std::min(total_s + std::min<uint64_t>(nS[i], uint64_t(1) << 45), uint64_t(1) << 45);
+    }
+    // a batch is judged by its totals; an instance with a tuner entry keeps
+    // the entry's choice
+    const bool nphj = auto_no_partition(cfg, total_r, total_s, kAutoNphjMaxBatchR);
     for (int i = 0; i < n; i++) {
         int rc = validate_sizes(nR[i], nS[i]);
         if (rc) return rc;
@@ -392,8 +426,8 @@ int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR
         s.nR = (uint32_t)nR[i];
         s.nS = (uint32_t)nS[i];
         s.cfg = resolve(cfg, nR[i], nS[i], -1.0);
-        if (nphj) s.cfg.alg = SPJ_ALG_NPHJ;
-        rc = check_resolved(s.cfg);
+        if (nphj && !tuner_has_entry((uint32_t)nR[i], (uint32_t)nS[i], -1.0)) s.cfg.alg = SPJ_ALG_NPHJ;
+        rc = check_resolved(s.cfg, nR[i]);
         if (rc) return rc;
     }
     std::lock_guard<std::mutex> lk(default_context_mutex());

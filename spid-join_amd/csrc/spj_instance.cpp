@@ -174,7 +174,7 @@ int spj_prepare(spj_instance_t *in, const spj_join_design_t *d)
     if (e == hipSuccess) e = in->dOut.ensure((size_t)in->nS * 8 + 8);
     if (e != hipSuccess) return hip_fail(e, "PrepareJoin allocation");
     in->spec.cfg = design_params(in, d);
-    int rc = check_resolved(in->spec.cfg);
+    int rc = check_resolved(in->spec.cfg, (uint64_t)in->nR);
     if (rc) return rc;
     in->prepared = true;
     return SPJ_OK;

@@ -70,6 +70,13 @@ Resolved tuned_params(uint32_t nR, uint32_t nS, double skew_hint)
     return r;
 }
 
+bool tuner_has_entry(uint32_t nR, uint32_t nS, double skew_hint)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_table.count(bucket(nR, nS, skew_hint))) return true;
+    return skew_hint >= 0 && g_table.count(bucket(nR, nS, 0.0));
+}
+
 }  // namespace spj
 
 using namespace spj;
@@ -91,11 +98,15 @@ int spj_tuner_set(uint64_t nR, uint64_t nS, double zipf, const spj_config_t *cfg
     return SPJ_OK;
 }
 
-/* Parameters that an instance of this shape would run with. */
+/* Parameters that a single call (spj_join_device with a default config) on
+ * relations of this shape would run with: the bucket's entry, else AUTO's
+ * choice (the no-partition join for |R| <= 2^20) with the defaults. */
 int spj_tuner_get(uint64_t nR, uint64_t nS, double zipf, spj_config_t *cfg)
 {
     if (!cfg) return fail(SPJ_EINVAL, "null config");
     Resolved r = tuned_params((uint32_t)nR, (uint32_t)nS, zipf);
+    if (!tuner_has_entry((uint32_t)nR, (uint32_t)nS, zipf) && auto_no_partition_single(nullptr, nR, nS))
+        r.alg = SPJ_ALG_NPHJ;
     cfg->algorithm = (int32_t)r.alg;
     cfg->radix_bits = (int32_t)r.bits;
     cfg->table_log2 = (int32_t)r.table_log2;
@@ -168,38 +179,37 @@ int spj_tuner_load(const char *path)
     return SPJ_OK;
 }
 
-/* Time candidate configurations for the bucket of (R, S, zipf) on the given
- * device-resident sample relations and keep the fastest.  Candidates vary
- * radix bits around the default (-2..+1), scatter tuples per thread (8, 12)
- * and LDS table bucket count (2^12, 2^13).  reps timed runs per candidate (median).
+/* Tune the bucket of (R, S, zipf) on the given device-resident relations and
+ * keep the fastest configuration.  Coordinate descent from the defaults: one
+ * parameter at a time takes each of its candidate values with the others at
+ * their best so far, in the order
+ *   algorithm      partitioned hash join, or the no-partition join (|R| <= 2^24)
+ *   radix bits     default - 2 .. default + 1 (R partitions within 4 LDS tables)
+ *   scatter ipt    8, 10, 12 tuples per thread per scatter round
+ *   table buckets  2^12, 2^13
+ *   S per item     2^16, 2^17, 2^18 S tuples per join work item
+ * and the sweep runs twice.  Each candidate runs reps + 1 times (the first
+ * is a warm-up); its score is the median device time (events around the
+ * launch sequence).  Workgroup sizes are not searched: the scatter and join
+ * kernels run one 1024-thread workgroup per CU that holds the whole LDS.
  * best_ms receives the winner's median time. */
 int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out, double zipf,
                  int reps, float *best_ms)
 {
     if (!R || !S || !out || nR == 0 || nS == 0) return fail(SPJ_EINVAL, "bad autotune arguments");
+    int rc = validate_sizes(nR, nS);
+    if (rc) return rc;
     if (reps < 1) reps = 1;
-    Resolved base = default_params((uint32_t)nR, (uint32_t)nS);
-    std::vector<Resolved> cands;
-    for (int db = -2; db <= 1; db++) {
-        int b = (int)base.bits + db;
-        if (b < 0 || b > (int)kMaxBits) continue;
-        for (uint32_t ipt : {8u, 12u}) {
-            for (uint32_t tl : {12u, 13u}) {
-                Resolved r = base;
-                r.bits = (uint32_t)b;
-                r.ipt = ipt;
-                r.table_log2 = tl;
-                // R partitions must fit the LDS table in at most a few sub-tables
-                uint64_t per = nR >> r.bits;
-                if (per > 4ull * join_table_cap()) continue;
-                cands.push_back(r);
-            }
-        }
-    }
-    float best = 1e30f;
-    Resolved win = base;
+    const Resolved base = default_params((uint32_t)nR, (uint32_t)nS);
     std::lock_guard<std::mutex> lk(default_context_mutex());
-    for (const Resolved &c : cands) {
+    std::map<std::vector<uint32_t>, float> seen;   // candidate -> median ms
+    auto score = [&](const Resolved &c, float *ms) -> int {
+        const std::vector<uint32_t> key = {c.alg, c.bits, c.ipt, c.table_log2, c.smax};
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+            *ms = it->second;
+            return SPJ_OK;
+        }
         InstSpec s;
         s.R = reinterpret_cast<const uint2 *>(R);
         s.S = reinterpret_cast<const uint2 *>(S);
@@ -207,23 +217,57 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
         s.nR = (uint32_t)nR;
         s.nS = (uint32_t)nS;
         s.cfg = c;
-        s.cfg.heavy = default_params((uint32_t)nR, (uint32_t)nS).heavy;
         std::vector<float> ts;
         std::vector<uint32_t> counts, errs;
         for (int i = 0; i < reps + 1; i++) {
             PhaseTimes t;
             hipError_t e = default_context().run({s}, counts, errs, &t);
             if (e != hipSuccess) return hip_fail(e, "autotune run");
-            if (errs[0]) break;
-            if (i) ts.push_back(t.total);   // first run is a warm-up
+            if (errs[0]) break;   // e.g. a duplicate key: no score
+            if (i) ts.push_back(t.total);
         }
-        if (ts.empty()) continue;
-        std::sort(ts.begin(), ts.end());
-        float med = ts[ts.size() / 2];
-        if (med < best) {
-            best = med;
-            win = c;
+        float med = 1e30f;
+        if (!ts.empty()) {
+            std::sort(ts.begin(), ts.end());
+            med = ts[ts.size() / 2];
         }
+        seen[key] = med;
+        *ms = med;
+        return SPJ_OK;
+    };
+    Resolved win = base;
+    float best = 1e30f;
+    rc = score(win, &best);
+    if (rc) return rc;
+    auto try_dim = [&](auto set, std::initializer_list<uint32_t> vals) -> int {
+        for (uint32_t v : vals) {
+            Resolved c = win;
+            set(c, v);
+            if (c.alg == SPJ_ALG_PHJ && (nR >> c.bits) > 4ull * join_table_cap()) continue;
+            if (c.alg == SPJ_ALG_NPHJ && nR > (uint64_t(1) << 24)) continue;
+            float ms = 0;
+            int r = score(c, &ms);
+            if (r) return r;
+            if (ms < best) {
+                best = ms;
+                win = c;
+            }
+        }
+        return SPJ_OK;
+    };
+    for (int sweep = 0; sweep < 2; sweep++) {
+        if ((rc = try_dim([](Resolved &c, uint32_t v) { c.alg = v; }, {(uint32_t)SPJ_ALG_PHJ, (uint32_t)SPJ_ALG_NPHJ})))
+            return rc;
+        if (win.alg == SPJ_ALG_NPHJ) break;   // no partition parameters to tune
+        const uint32_t b0 = base.bits;
+        std::vector<uint32_t> bits;
+        for (int d = -2; d <= 1; d++)
+            if ((int)b0 + d >= 0 && (int)b0 + d <= (int)kMaxBits) bits.push_back(b0 + d);
+        for (uint32_t b : bits)
+            if ((rc = try_dim([](Resolved &c, uint32_t v) { c.bits = v; }, {b}))) return rc;
+        if ((rc = try_dim([](Resolved &c, uint32_t v) { c.ipt = v; }, {8u, 10u, 12u}))) return rc;
+        if ((rc = try_dim([](Resolved &c, uint32_t v) { c.table_log2 = v; }, {12u, 13u}))) return rc;
+        if ((rc = try_dim([](Resolved &c, uint32_t v) { c.smax = v; }, {1u << 16, 1u << 17, 1u << 18}))) return rc;
     }
     {
         std::lock_guard<std::mutex> lk2(g_mu);

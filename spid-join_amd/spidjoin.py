@@ -215,6 +215,20 @@ def checksum_device(ptr, n):
     return s.value, x.value
 
 
+def autotune(dR, nR, dS, nS, dOut, zipf, reps=3):
+    """spj_autotune on device relations; returns the winner's median ms."""
+    ms = ctypes.c_float()
+    check(lib().spj_autotune(dR.ptr, nR, dS.ptr, nS, dOut.ptr, float(zipf), int(reps), ctypes.byref(ms)))
+    return ms.value
+
+
+def tuner_get(nR, nS, zipf):
+    """The parameters a single call on relations of this shape would run with."""
+    c = Config()
+    check(lib().spj_tuner_get(nR, nS, float(zipf), ctypes.byref(c)))
+    return {n: getattr(c, n) for n, _ in Config._fields_}
+
+
 def graph_stats():
     """(captures, replays) of launch sequences replayed as HIP graphs."""
     c, r = U64(), U64()

@@ -211,8 +211,9 @@ def test_no_partition_hash_join_batch_and_instance(gpu, orc):
 
 def test_auto_picks_no_partition_join_for_small_calls(gpu, orc):
     """AUTO runs the no-partition join (no partition pass: r_partitioned 0)
-    when a call's R relations total at most 2^20 tuples and no partition
-    parameter is set; otherwise the partitioned join."""
+    for a small call (one join of |R| <= 2^21 and |R| |S| <= 2^44, a batch
+    of total |R| <= 2^20) with no partition parameter set; otherwise the
+    partitioned join."""
     sj = gpu
     R = sj.gen_r(1 << 20, 171)
     S = sj.gen_s(1 << 21, R, 1.2, 172)
@@ -222,11 +223,13 @@ def test_auto_picks_no_partition_join_for_small_calls(gpu, orc):
         pairs, t = sj.join(R, S, cfg, timed=True)
         check_equal(sj, orc, R, S, pairs)
         assert (t["r_partitioned"] == len(R)) == parted, (cfg, t)
-    R2 = sj.gen_r((1 << 20) + 1, 173)
-    S2 = sj.gen_s(1 << 21, R2, 0.5, 174)
-    pairs, t = sj.join(R2, S2, timed=True)
-    check_equal(sj, orc, R2, S2, pairs)
-    assert t["r_partitioned"] == len(R2)
+    # above 2^21 R tuples, or |R| |S| above 2^44: partitioned
+    for nr, ns in (((1 << 21) + 1, 1 << 22), (1 << 19, (1 << 25) + 64)):
+        R2 = sj.gen_r(nr, 173)
+        S2 = sj.gen_s(ns, R2, 0.5, 174)
+        pairs, t = sj.join(R2, S2, timed=True)
+        check_equal(sj, orc, R2, S2, pairs)
+        assert t["r_partitioned"] == len(R2), (nr, ns)
     # a batch is judged by its total |R|
     Rs = [sj.gen_r(1 << 18, 180 + i) for i in range(5)]
     Ss = [sj.gen_s(1 << 19, Rs[i], 0.8, 190 + i) for i in range(5)]
