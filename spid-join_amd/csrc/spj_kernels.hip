@@ -609,16 +609,30 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // tile switch that asked for it, before the next switch reads it)
         uint32_t qv = 0;
         if (qwant) qv = atomicAdd(q, 1u);
-        uint32_t cb[FMAX / NT], cc[FMAX / NT];
+        // Two adjacent digits per atomic: their cursors form one 64-bit word
+        // (group counters start at even offsets), and the low cursor never
+        // carries into the high one (cursors stay below 2^32).  Thread t
+        // takes digits 2t, 2t + 1, 2(NT + t), 2(NT + t) + 1 (slot u of cb/cc
+        // is digit 2((u >> 1) NT + t) + (u & 1)).
+        uint32_t cc[FMAX / NT];
+        unsigned long long cp[FMAX / NT / 2];
 #pragma unroll
-        for (uint32_t u = 0; u < FMAX / NT; u++) {
-            const uint32_t d = u * NT + threadIdx.x;
-            cb[u] = 0;
-            cc[u] = 0;
-            if (d < F) {
-                cc[u] = lst[d];
-                if (cc[u]) cb[u] = atomicAdd(&cur[d], cc[u]);
+        for (uint32_t h = 0; h < FMAX / NT / 2; h++) {
+            const uint32_t d0 = 2 * (h * NT + threadIdx.x);
+            cp[h] = 0;
+            cc[2 * h] = d0 < F ? lst[d0] : 0u;
+            cc[2 * h + 1] = d0 + 1 < F ? lst[d0 + 1] : 0u;   // F == 1: no second digit
+        }
+        if (F >= 2) {
+#pragma unroll
+            for (uint32_t h = 0; h < FMAX / NT / 2; h++) {
+                const uint32_t d0 = 2 * (h * NT + threadIdx.x);
+                if (d0 < F && (cc[2 * h] | cc[2 * h + 1]))
+                    cp[h] = atomicAdd(reinterpret_cast<unsigned long long *>(cur + d0),
+                                      (unsigned long long)cc[2 * h] | ((unsigned long long)cc[2 * h + 1] << 32));
             }
+        } else if (threadIdx.x == 0 && cc[0]) {
+            cp[0] = atomicAdd(&cur[0], cc[0]);
         }
         const uint32_t total = block_exclusive_scan(lst, F + 1, wtmp);   // lst[d] = start of digit d
         const uint32_t hstart = lst[F];                                  // == total without heavy tuples
@@ -642,9 +656,9 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         __syncthreads();   // staging has read the starts; lst[d] becomes digit d's base - start
 #pragma unroll
         for (uint32_t u = 0; u < FMAX / NT; u++) {
-            const uint32_t d = u * NT + threadIdx.x;
+            const uint32_t d = 2 * ((u >> 1) * NT + threadIdx.x) + (u & 1u);
             if (d < F) {
-                uint32_t b = cb[u];
+                uint32_t b = (uint32_t)(cp[u >> 1] >> ((u & 1u) * 32));
                 if (td.cap != 0 && cc[u] != 0 && b + cc[u] > td.cell0 + (d + 1) * td.cap) {
                     // the cell is full: the run goes to the sink and the
                     // join is redone with counted partitions
@@ -957,7 +971,7 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
 // partition's SMap entry strings together; a cell's fill is clamped to its
 // capacity (an overflowed cell has been flagged, and the join is redone).
 // spart[inst] and rpart[inst] (zeroed before) receive the number of
-// partitioned S and R tuples.  Grid (instances, partition chunks of 1024).
+// partitioned S and R tuples.  Grid (instances, partition chunks of kItemsNT).
 // ---------------------------------------------------------------------------
 // Partition p's position map from its cells' fills (clamped to the
 // capacity); returns the partition's tuple count.
@@ -984,21 +998,25 @@ __device__ __forceinline__ void identity_map(SMap &m)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
+// 256 partitions per block: the item words (160 bytes per partition) are
+// written by 16 blocks per instance at 2^12 partitions, not 4.
+constexpr uint32_t kItemsNT = 256;
+
+__global__ __launch_bounds__(kItemsNT) void k_make_items(const InstDesc *__restrict__ insts, const uint32_t *__restrict__ pstart,
                                                      const CellGroup *__restrict__ cells,
                                                      const uint32_t *__restrict__ counts,
                                                      uint32_t smax, WorkItem *__restrict__ items,
                                                      uint32_t *__restrict__ nitems, uint32_t *__restrict__ spart,
                                                      uint32_t *__restrict__ rpart)
 {
-    __shared__ uint32_t wtmp[1024 / kWave + 1];
-    __shared__ uint32_t cnt[1024];
+    __shared__ uint32_t wtmp[kItemsNT / kWave + 1];
+    __shared__ uint32_t cnt[kItemsNT];
     __shared__ uint32_t base;
     const uint32_t inst = blockIdx.x;
     const InstDesc &I = insts[inst];
     const uint32_t F = 1u << I.bits;
     const uint32_t *pr = pstart + I.pstartR, *ps = pstart + I.pstartS;
-    const uint32_t p0 = blockIdx.y * 1024u;
+    const uint32_t p0 = blockIdx.y * kItemsNT;
     if (p0 >= F) return;   // whole block
     {
         const uint32_t p = p0 + threadIdx.x;
@@ -1013,7 +1031,7 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
         }
         cnt[threadIdx.x] = c;
         __syncthreads();
-        const uint32_t tot = block_exclusive_scan(cnt, 1024, wtmp);
+        const uint32_t tot = block_exclusive_scan(cnt, kItemsNT, wtmp);
         if (threadIdx.x == 0) base = tot ? atomicAdd(nitems, tot) : 0u;
         __syncthreads();
         uint32_t o = base + cnt[threadIdx.x];
@@ -1043,10 +1061,10 @@ __global__ __launch_bounds__(1024) void k_make_items(const InstDesc *__restrict_
         __syncthreads();
         cnt[threadIdx.x] = ns;
         __syncthreads();
-        const uint32_t all_s = block_exclusive_scan(cnt, 1024, wtmp);
+        const uint32_t all_s = block_exclusive_scan(cnt, kItemsNT, wtmp);
         cnt[threadIdx.x] = nr;
         __syncthreads();
-        const uint32_t all_r = block_exclusive_scan(cnt, 1024, wtmp);
+        const uint32_t all_r = block_exclusive_scan(cnt, kItemsNT, wtmp);
         if (threadIdx.x == 0) {
             if (all_s) atomicAdd(&spart[inst], all_s);
             if (all_r) atomicAdd(&rpart[inst], all_r);
@@ -2173,7 +2191,7 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
                              uint32_t *spart, uint32_t *rpart, hipStream_t st)
 {
     if (!ninst) return hipSuccess;
-    hipLaunchKernelGGL(k_make_items, dim3(ninst, (1u << kMaxBits) / 1024u), dim3(1024), 0, st, insts, pstart, cells,
+    hipLaunchKernelGGL(k_make_items, dim3(ninst, (1u << kMaxBits) / kItemsNT), dim3(kItemsNT), 0, st, insts, pstart, cells,
                        counts, smax, items, nitems, spart, rpart);
     return hipGetLastError();
 }
