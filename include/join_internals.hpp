@@ -48,6 +48,10 @@ public:
     void StartJoin(join_design_t join_design);
     ResultBuffers_t FinishJoin();
     void RecordLogs(struct timespec start, int rank_group);
+    // Not in the reference: FinishJoin's buffers.  SPJ_RESULT_EXACT (default)
+    // gives one exact buffer; SPJ_RESULT_RANK_PADDED the reference's layout,
+    // one zero-padded buffer per rank (see spidjoin.h).
+    void SetResultLayout(int layout);
 
 private:
     spj_instance_t *h_;

@@ -185,6 +185,22 @@ int spj_start(spj_instance_t *inst, const spj_join_design_t *design);
 /* FinishJoin, query.cc:193: wait, copy exactly *count pairs into a library-
  * allocated host buffer the caller frees with spj_free_result. */
 int spj_finish(spj_instance_t *inst, spj_pair_t **pairs, uint64_t *count);
+/* FinishJoin's buffers (ResultBuffers_t, type_defs.hpp:82), by layout:
+ *   SPJ_RESULT_EXACT (default): one buffer of exactly `count` pairs;
+ *   SPJ_RESULT_RANK_PADDED: the reference's gathered layout
+ *     (queryHostFunc.cc:1058-1126, finish_join.c:48-141): one buffer per rank
+ *     (num_rank_allocated of them), each 64 DPU blocks of the rank's largest
+ *     block size; a block holds the pairs whose S tuple's global partition
+ *     glb_partition_hash(key) & (64 ranks - 1) is that DPU, then (0, 0)
+ *     pairs; byte counts include the padding.  Plain plan only (bank_set =
+ *     rank_set = 1, join_alg 0, 2 or 3), ranks a power of two <= 64; heavy-
+ *     key handling is off in this layout.  (0, 0) is also a legal pair when
+ *     rid 0 occurs on both sides, which is why it is not the default. */
+enum { SPJ_RESULT_EXACT = 0, SPJ_RESULT_RANK_PADDED = 1 };
+int spj_set_result_layout(spj_instance_t *inst, int layout);
+/* FinishJoin in either layout: up to cap buffers (library-allocated, freed
+ * with spj_free_result) and their byte counts; *nbufs receives the number. */
+int spj_finish_buffers(spj_instance_t *inst, spj_pair_t **bufs, uint64_t *bytes, int cap, int *nbufs);
 /* ExecuteJoin("phj" | "smj", design), query.cc:199: start + finish. */
 int spj_execute(spj_instance_t *inst, const char *join_type, const spj_join_design_t *design, spj_pair_t **pairs,
                 uint64_t *count);

@@ -149,7 +149,7 @@ Context::~Context()
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
     DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
-                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &tiles_};
+                      &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &tiles_, &nphj_, &ext_};
     for (auto *b : bufs) b->release();
     pin_desc_.release();
     pin_ctrl_.release();
@@ -622,8 +622,9 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
     const DeviceInfo &di = device_info();
     const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
-    SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err,
-                        holes_.as<uint4>(), max_holes_, grid, st));
+    if (record_ext_) SPJ_TRY(ext_.ensure(items_bound * sizeof(uint4)));
+    SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err, holes_.as<uint4>(), max_holes_,
+                        record_ext_ ? ext_.as<uint4>() : nullptr, grid, st));
     SPJ_TRY(launch_nphj(d_insts, reinterpret_cast<const NphjJob *>(dd + o_nphj), (uint32_t)h_nphj_.size(),
                         nphj_.as<uint64_t>(), nphj_words, (uint32_t)nphj_bb, (uint32_t)nphj_pb, out_lo, out_fill,
                         err, st));
@@ -728,6 +729,24 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
             timed->r_cells = d0.r_cells != 0 ? 1u : 0u;
         }
     }
+    return hipSuccess;
+}
+
+void Context::set_record_extents(bool on)
+{
+    if (on == record_ext_) return;
+    record_ext_ = on;
+    drop_graph();   // the launch sequence changes
+    last_valid_ = false;
+}
+
+hipError_t Context::item_extents(std::vector<uint4> &ext)
+{
+    ext.clear();
+    if (!record_ext_ || h_ctrl_.empty()) return hipErrorInvalidValue;
+    const uint32_t n = h_ctrl_[0];   // work items of the last collected batch
+    ext.resize(n);
+    if (n) SPJ_TRY(hipMemcpy(ext.data(), ext_.p, (size_t)n * sizeof(uint4), hipMemcpyDeviceToHost));
     return hipSuccess;
 }
 

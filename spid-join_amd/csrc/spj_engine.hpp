@@ -104,6 +104,12 @@ public:
     uint64_t graph_captures() const { return graph_captures_; }
     uint64_t graph_replays() const { return graph_replays_; }
 
+    // Record, for every join work item, (instance, partition, first output
+    // position, pairs written); read them with item_extents() after collect().
+    // Positions are before the compaction of holes (see collect).
+    void set_record_extents(bool on);
+    hipError_t item_extents(std::vector<uint4> &ext);
+
     // After run()/collect() of a batch, copy instance i's partitioned R and
     // its F + 1 partition starts to caller-owned device buffers.
     hipError_t export_partitioned_r(size_t inst, uint2 *rp_dst, uint32_t *pstart_dst);
@@ -123,7 +129,8 @@ private:
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;
-    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_, nphj_;
+    DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_, nphj_, ext_;
+    bool record_ext_ = false;
     PinBuf pin_desc_, pin_ctrl_;
     hipEvent_t desc_ev_ = nullptr;   // the last descriptor upload has left pin_desc_
     bool desc_pending_ = false;

@@ -13,6 +13,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -41,9 +42,19 @@ struct spj_instance {
     hipEvent_t ev[3] = {};
     int64_t t0_ns = 0, recv0_ns = 0, recv1_ns = 0;
     std::vector<spj::NodeTime> node_times;
+    int layout = SPJ_RESULT_EXACT;   // FinishJoin's result buffers
 };
 
 using namespace spj;
+
+// log2 of the reference's DPU count, 64 per rank (NUM_DPU_RANK), ranks a
+// power of two as the reference's rank allocation (idpHandler.hpp:134)
+static uint32_t rank_dpu_bits(int ranks)
+{
+    uint32_t b = 6;
+    while ((1 << (b - 6)) < ranks) b++;
+    return b;
+}
 
 static int64_t now_ns()
 {
@@ -189,6 +200,26 @@ int spj_start(spj_instance_t *in, const spj_join_design_t *d)
     }
     if (in->started) return fail(SPJ_ESTATE, "StartJoin twice without FinishJoin");
     in->spec.cfg = design_params(in, d);
+    if (in->layout == SPJ_RESULT_RANK_PADDED) {
+        // the reference's per-DPU result blocks: a pair belongs to the DPU of
+        // its S tuple's global partition, glb_partition_hash(key) & (D - 1)
+        // with D = 64 ranks DPUs (glb_partition_count.c:192-195) -- the low
+        // bits of this build's own partition digit, given at least log2 D
+        // radix bits.  Heavy pairs bypass the partitions, so heavy handling
+        // is off; the replicating plans place S by position, which has no
+        // counterpart here.
+        if (design_replicates(d) || (in->plan_set && in->plan.replicate))
+            return fail(SPJ_EINVAL, "the rank-padded result layout covers the plain plan (bank_set = rank_set = 1)");
+        if (in->spec.cfg.alg == SPJ_ALG_NPHJ)
+            return fail(SPJ_EINVAL, "the rank-padded result layout needs a partitioned join (join_alg 0, 2 or 3)");
+        if (in->ranks < 1 || (in->ranks & (in->ranks - 1)))
+            return fail(SPJ_EINVAL, "the rank-padded result layout needs a power-of-two rank count, got %d", in->ranks);
+        const uint32_t dbits = rank_dpu_bits(in->ranks);
+        if (dbits > kMaxBits) return fail(SPJ_EINVAL, "the rank-padded result layout takes at most 64 ranks");
+        in->spec.cfg.bits = std::max(in->spec.cfg.bits, dbits);
+        in->spec.cfg.heavy = 0;
+    }
+    in->ctx.set_record_extents(in->layout == SPJ_RESULT_RANK_PADDED);
     in->run_plan = in->plan_set ? in->plan : spj::default_plan(design_replicates(d));
     hipStream_t st = in->ctx.stream();
     hipError_t e = hipSuccess;
@@ -216,23 +247,66 @@ int spj_start(spj_instance_t *in, const spj_join_design_t *d)
     return SPJ_OK;
 }
 
+static int finish_exact(spj_instance *in, spj_pair_t **pairs, uint64_t *count);
+static int finish_rank_padded(spj_instance *in, spj_pair_t **bufs, uint64_t *bytes, int cap, int *nbufs);
+
+int spj_set_result_layout(spj_instance_t *in, int layout)
+{
+    if (!in) return fail(SPJ_EINVAL, "null instance");
+    if (layout != SPJ_RESULT_EXACT && layout != SPJ_RESULT_RANK_PADDED) return fail(SPJ_EINVAL, "layout %d", layout);
+    if (in->started) return fail(SPJ_ESTATE, "layout change while a join is running");
+    in->layout = layout;
+    return SPJ_OK;
+}
+
+int spj_finish_buffers(spj_instance_t *in, spj_pair_t **bufs, uint64_t *bytes, int cap, int *nbufs)
+{
+    if (!in || !bufs || !bytes || !nbufs) return fail(SPJ_EINVAL, "null argument");
+    if (!in->started) return fail(SPJ_ESTATE, "FinishJoin before StartJoin");
+    if (in->layout == SPJ_RESULT_RANK_PADDED) return finish_rank_padded(in, bufs, bytes, cap, nbufs);
+    if (cap < 1) return fail(SPJ_EINVAL, "room for one buffer needed");
+    uint64_t n = 0;
+    int rc = finish_exact(in, &bufs[0], &n);
+    if (rc) return rc;
+    bytes[0] = n * sizeof(spj_pair_t);
+    *nbufs = 1;
+    return SPJ_OK;
+}
+
 int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
 {
     if (!in || !pairs || !count) return fail(SPJ_EINVAL, "null argument");
     if (!in->started) return fail(SPJ_ESTATE, "FinishJoin before StartJoin");
+    if (in->layout != SPJ_RESULT_EXACT)
+        return fail(SPJ_ESTATE, "the instance returns rank-padded buffers: use spj_finish_buffers");
+    return finish_exact(in, pairs, count);
+}
+
+// Wait for the join, record its timeline, check its flags.
+static int finish_wait(spj_instance *in, uint64_t *count, float *send_r, float *send_s)
+{
     in->started = false;
     std::vector<uint32_t> counts, errs;
     hipError_t e = in->ctx.collect(1, counts, errs, &in->times);
     if (e != hipSuccess) return hip_fail(e, "FinishJoin");
     in->have_times = true;
-    float send_r = 0, send_s = 0;
-    e = hipEventElapsedTime(&send_r, in->ev[0], in->ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&send_s, in->ev[1], in->ev[2]);
+    e = hipEventElapsedTime(send_r, in->ev[0], in->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(send_s, in->ev[1], in->ev[2]);
     if (e != hipSuccess) return hip_fail(e, "FinishJoin timing");
-    plan_times(in, send_r, send_s);
+    plan_times(in, *send_r, *send_s);
     int rc = err_from_flags(errs[0], 0);
     if (rc) return rc;
-    uint64_t n = counts[0];
+    *count = counts[0];
+    return SPJ_OK;
+}
+
+static int finish_exact(spj_instance *in, spj_pair_t **pairs, uint64_t *count)
+{
+    uint64_t n = 0;
+    float send_r = 0, send_s = 0;
+    int rc = finish_wait(in, &n, &send_r, &send_s);
+    if (rc) return rc;
+    hipError_t e;
     // aligned_alloc like the reference's result buffers (queryHostFunc.cc:1089)
     spj_pair_t *buf = (spj_pair_t *)alloc_result_buffer(n * 8);
     if (!buf) return fail(SPJ_ENOMEM, "result buffer");
@@ -248,6 +322,71 @@ int spj_finish(spj_instance_t *in, spj_pair_t **pairs, uint64_t *count)
     plan_times(in, send_r, send_s);
     *pairs = buf;
     *count = n;
+    return SPJ_OK;
+}
+
+// The reference's gathered result (queryHostFunc.cc:1058-1126,
+// finish_join.c:48-141): one buffer per rank of 64 DPU blocks, each block
+// the DPU's pairs followed by (0, 0) pairs up to the rank's largest block;
+// the byte count includes the padding.  A pair's DPU is its S tuple's
+// global partition modulo D = 64 ranks (the join ran with at least log2 D
+// radix bits, so every work item belongs to one DPU).  Pairs inside a block
+// are in no particular order, as in the reference.
+static int finish_rank_padded(spj_instance *in, spj_pair_t **bufs, uint64_t *bytes, int cap, int *nbufs)
+{
+    const int ranks = in->ranks;
+    if (cap < ranks) return fail(SPJ_EINVAL, "room for %d buffers needed", ranks);
+    uint64_t n = 0;
+    float send_r = 0, send_s = 0;
+    int rc = finish_wait(in, &n, &send_r, &send_s);
+    if (rc) return rc;
+    std::vector<uint4> ext;
+    hipError_t e = in->ctx.item_extents(ext);
+    if (e != hipSuccess) return hip_fail(e, "FinishJoin item extents");
+    in->recv0_ns = now_ns();
+    std::vector<spj_pair_t> all(n);
+    if (n) {
+        e = stage_d2h(all.data(), in->dOut.p, n * 8, in->ctx.stream());
+        if (e != hipSuccess) return hip_fail(e, "FinishJoin D2H");
+    }
+    // the result was compacted: the items' pairs lie back to back in the
+    // order of their reservations
+    std::sort(ext.begin(), ext.end(), [](const uint4 &a, const uint4 &b) { return a.z < b.z; });
+    const uint32_t D = 64u * (uint32_t)ranks;
+    std::vector<uint64_t> per_dpu(D, 0);
+    uint64_t pos = 0;
+    for (const uint4 &x : ext) {
+        per_dpu[x.y & (D - 1)] += x.w;
+        pos += x.w;
+    }
+    if (pos != n) return fail(SPJ_EHIP, "FinishJoin: item extents cover %llu of %llu pairs", (unsigned long long)pos,
+                              (unsigned long long)n);
+    std::vector<uint64_t> block(ranks, 0), fill(D, 0);
+    for (uint32_t d = 0; d < D; d++) block[d / 64] = std::max(block[d / 64], per_dpu[d]);
+    int made = 0;
+    for (int r = 0; r < ranks; r++) {
+        const uint64_t b = 64 * block[r] * sizeof(spj_pair_t);
+        spj_pair_t *buf = (spj_pair_t *)alloc_result_buffer(b ? b : 64);
+        if (!buf) {
+            for (int k = 0; k < made; k++) free(bufs[k]);
+            return fail(SPJ_ENOMEM, "result buffer");
+        }
+        memset(buf, 0, b);   // the (0, 0) padding of finish_join.c
+        bufs[r] = buf;
+        bytes[r] = b;
+        made++;
+    }
+    pos = 0;
+    for (const uint4 &x : ext) {
+        const uint32_t d = x.y & (D - 1), r = d / 64;
+        spj_pair_t *dst = bufs[r] + (uint64_t)(d % 64) * block[r] + fill[d];
+        if (x.w) memcpy(dst, all.data() + pos, (size_t)x.w * sizeof(spj_pair_t));
+        fill[d] += x.w;
+        pos += x.w;
+    }
+    in->recv1_ns = now_ns();
+    plan_times(in, send_r, send_s);
+    *nbufs = ranks;
     return SPJ_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "pidjoin.hpp"
 
@@ -71,13 +72,22 @@ void JoinInstance::StartJoin(join_design_t d)
 
 ResultBuffers_t JoinInstance::FinishJoin()
 {
-    spj_pair_t *pairs = nullptr;
-    uint64_t n = 0;
-    check(spj_finish(h_, &pairs, &n), "FinishJoin");
-    result_bufferpool.first.assign(1, reinterpret_cast<kv_pair_t *>(pairs));
-    result_bufferpool.second.assign(1, (int64_t)(n * sizeof(kv_pair_t)));
+    // one exact buffer, or one zero-padded buffer per rank (SetResultLayout)
+    const int cap = num_rank_allocated > 1 ? num_rank_allocated : 1;
+    std::vector<spj_pair_t *> bufs((size_t)cap, nullptr);
+    std::vector<uint64_t> bytes((size_t)cap, 0);
+    int n = 0;
+    check(spj_finish_buffers(h_, bufs.data(), bytes.data(), cap, &n), "FinishJoin");
+    result_bufferpool.first.resize((size_t)n);
+    result_bufferpool.second.resize((size_t)n);
+    for (int i = 0; i < n; i++) {
+        result_bufferpool.first[(size_t)i] = reinterpret_cast<kv_pair_t *>(bufs[(size_t)i]);
+        result_bufferpool.second[(size_t)i] = (int64_t)bytes[(size_t)i];
+    }
     return result_bufferpool;
 }
+
+void JoinInstance::SetResultLayout(int layout) { check(spj_set_result_layout(h_, layout), "SetResultLayout"); }
 
 void JoinInstance::ExecuteJoin(std::string join_type, join_design_t d)
 {

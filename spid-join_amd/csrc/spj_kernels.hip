@@ -1249,7 +1249,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                                   const WorkItem *__restrict__ items, uint32_t *__restrict__ ctrl,
                                                   uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill,
                                                   uint32_t *__restrict__ err, uint4 *__restrict__ holes,
-                                                  uint32_t max_holes)
+                                                  uint32_t max_holes, uint4 *__restrict__ ext)
 {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ JoinShared sh;
@@ -1882,6 +1882,9 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                 const uint32_t h = atomicAdd(&ctrl[2], 1u);
                 if (h < max_holes) holes[h] = make_uint4(w.inst, sh.base + sh.fill, sh.res - sh.fill, 0u);
             }
+            // optionally, where the item's pairs went: (instance, partition,
+            // first position, count), for results grouped by partition
+            if (ext) ext[it] = make_uint4(w.inst, items[it].part, sh.base, sh.fill);   // (read here: nothing held across the item)
             if (!advanced) {   // (the hash path took the next item at its first sub-table)
                 sh.item = sh.item2;
                 sh.item2 = atomicAdd(&ctrl[1], 1u);
@@ -2198,13 +2201,13 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
 
 hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
-                       uint32_t grid, hipStream_t st)
+                       uint4 *ext, uint32_t grid, hipStream_t st)
 {
     size_t l = lds_join();
     hipError_t e = set_lds((const void *)k_join, l);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
-                       holes, max_holes);
+                       holes, max_holes, ext);
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "spj_partition_device", "spj_join_device_noalloc", "spj_graph_stats", "spj_join_host",
     "spj_join_batch_device", "spj_join_design_default", "spj_create", "spj_load_column", "spj_prepare",
     "spj_start", "spj_finish", "spj_execute", "spj_last_phase_times", "spj_set_query_plan", "spj_plan_check",
-    "spj_timeline", "spj_free_result", "spj_destroy",
+    "spj_timeline", "spj_set_result_layout", "spj_finish_buffers", "spj_free_result", "spj_destroy",
     "spj_tuner_set", "spj_tuner_get", "spj_tuner_clear", "spj_tuner_entries", "spj_tuner_save",
     "spj_tuner_load", "spj_autotune", "spj_gen_r", "spj_gen_s", "spj_server_create", "spj_server_add_table",
     "spj_server_submit", "spj_server_wait", "spj_server_stats", "spj_server_destroy",
@@ -112,6 +112,8 @@ def lib():
         "spj_plan_check": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "spj_timeline": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_char_p, U64,
                                         ctypes.POINTER(U64)]),
+        "spj_set_result_layout": (ctypes.c_int, [P, ctypes.c_int]),
+        "spj_finish_buffers": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "spj_free_result": (None, [P]),
         "spj_destroy": (None, [P]),
         "spj_tuner_set": (ctypes.c_int, [U64, U64, ctypes.c_double, ctypes.POINTER(Config)]),
@@ -364,6 +366,27 @@ class JoinInstance:
             return out
         finally:
             lib().spj_free_result(p)
+
+    RESULT_EXACT, RESULT_RANK_PADDED = 0, 1
+
+    def set_result_layout(self, layout):
+        check(lib().spj_set_result_layout(self.h, int(layout)))
+
+    def finish_buffers(self, cap=64):
+        """FinishJoin as buffers: a list of pair arrays, one per buffer (their
+        lengths are the byte counts / 8, padding included)."""
+        bufs = (P * cap)()
+        nbytes = (U64 * cap)()
+        n = ctypes.c_int()
+        check(lib().spj_finish_buffers(self.h, ctypes.cast(bufs, P), ctypes.cast(nbytes, P), cap, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            a = np.empty(nbytes[i] // 8, dtype=PAIR)
+            if nbytes[i]:
+                ctypes.memmove(a.ctypes.data, bufs[i], nbytes[i])
+            lib().spj_free_result(bufs[i])
+            out.append(a)
+        return out
 
     def phase_times(self):
         t = PhaseTimes()

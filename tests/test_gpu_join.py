@@ -669,3 +669,54 @@ def test_sort_merge_key_range_buckets(gpu, orc, layout):
         with pytest.raises(sj.SpjError) as e:
             sj.join(R3, S, cfg)
         assert e.value.code == -4
+
+
+@pytest.mark.parametrize("ranks,alg", [(16, 0), (4, 2), (8, 3)])
+def test_rank_padded_result_layout(gpu, orc, ranks, alg):
+    """FinishJoin in the reference's gathered layout: one buffer per rank of
+    64 DPU blocks, each the pairs whose S tuple's global partition
+    glb_partition_hash(key) & (64 ranks - 1) is that DPU, then (0, 0) pairs up
+    to the rank's largest block (queryHostFunc.cc:1058-1126, finish_join.c)."""
+    sj = gpu
+    nR = 1 << 18 if alg != 3 else 1 << 16
+    R = sj.gen_r(nR, 210 + ranks)
+    S = sj.gen_s(2 * nR, R, 0.6, 211 + ranks)
+    S["key"][::37] += np.uint32(1 << 24)   # unmatched: no pair, no slot
+    D = 64 * ranks
+    order = np.argsort(R["key"], kind="stable")
+    rk = R["key"][order]
+    pos = np.minimum(np.searchsorted(rk, S["key"]), len(rk) - 1)
+    hit = (rk[pos] == S["key"]) & (S["key"] != 0)
+    dpu = orc.hash_batch("glb_partition_hash", S["key"][hit]) & np.uint32(D - 1)
+    want = (R["rid"][order][pos[hit]].astype(np.uint64) << np.uint64(32)) | S["rid"][hit].astype(np.uint64)
+    count = np.bincount(dpu, minlength=D)
+    ji = sj.JoinInstance(ranks, 1, ranks)
+    ji.load_column(R, "left")
+    ji.load_column(S, "right")
+    ji.set_result_layout(sj.JoinInstance.RESULT_RANK_PADDED)
+    print(f"case rank-padded ranks={ranks} alg={alg}", flush=True)
+    ji.start(sj.JoinInstance.design(join_alg=alg))
+    bufs = ji.finish_buffers()
+    assert len(bufs) == ranks
+    by_dpu = np.argsort(dpu, kind="stable")
+    starts = np.concatenate([[0], np.cumsum(count)])
+    for r in range(ranks):
+        block = int(count[r * 64:(r + 1) * 64].max())
+        assert len(bufs[r]) == 64 * block, r
+        w = words(bufs[r])
+        for i in range(64):
+            d = r * 64 + i
+            c = int(count[d])
+            blk = w[i * block:(i + 1) * block]
+            assert np.array_equal(np.sort(blk[:c]), np.sort(want[by_dpu[starts[d]:starts[d + 1]]])), d
+            assert not blk[c:].any(), d
+    # the exact layout is back on request; replication and no-partition joins refuse the padded one
+    ji.set_result_layout(sj.JoinInstance.RESULT_EXACT)
+    ji.start(sj.JoinInstance.design(join_alg=alg))
+    check_equal(sj, orc, R, S, ji.finish())
+    ji.set_result_layout(sj.JoinInstance.RESULT_RANK_PADDED)
+    for d in (sj.JoinInstance.design(join_alg=0, bank_set=8), sj.JoinInstance.design(join_alg=1)):
+        with pytest.raises(sj.SpjError) as e:
+            ji.start(d)
+        assert e.value.code == -1
+    ji.close()
