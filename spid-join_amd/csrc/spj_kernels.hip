@@ -111,6 +111,21 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave, by DPP: shifts by 1, 2,
+// 4 and 8 lanes inside each row of 16 (lanes shifted in from outside the row
+// read 0), then lane 15 of row 0 added to row 1 and of row 2 to row 3
+// (row_bcast:15), then lane 31 added to rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1 and 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2 and 3
+    return x;
+}
+
 // In-place exclusive scan of a[0..n) in LDS by the whole workgroup; returns
 // the total.  wtmp needs blockDim/64 + 1 words.
 __device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wtmp)
@@ -127,22 +142,12 @@ __device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wtmp
     uint32_t b = tid * per, e = min(n, b + per);
     uint32_t s = 0;
     for (uint32_t i = b; i < e; i++) s += a[i];
-    uint32_t x = s;
-#pragma unroll
-    for (uint32_t off = 1; off < kWave; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, kWave);
-        if (lane >= off) x += y;
-    }
+    const uint32_t x = wave_incl_sum(s);
     if (lane == kWave - 1) wtmp[wid] = x;
     __syncthreads();
     if (tid < kWave) {
         uint32_t v = (tid < nw) ? wtmp[tid] : 0u;
-        uint32_t z = v;
-#pragma unroll
-        for (uint32_t off = 1; off < kWave; off <<= 1) {
-            uint32_t y = __shfl_up(z, off, kWave);
-            if (tid >= off) z += y;
-        }
+        const uint32_t z = wave_incl_sum(v);
         if (tid < nw) wtmp[tid] = z - v;
         if (tid == kWave - 1) wtmp[nw] = z;
     }
@@ -1253,7 +1258,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ JoinShared sh;
-    __shared__ uint32_t wtmp[kJoinNT / kWave + 1];
+    __shared__ __align__(16) uint32_t wtmp[kJoinNT / kWave + 1];
     constexpr uint32_t NT = kJoinNT, IPT = kJoinIPT, ROUND = NT * IPT;
     const uint32_t total_items = ctrl[0];
     // The queue runs two items ahead, so the next item's words can be
@@ -1470,13 +1475,19 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 
                     // LDS: (key, rid) pairs counting-sorted by bucket, then 16-bit
                     // bucket bounds: bend[0] = 0 and bend[b + 1] = end of bucket b, so
-                    // bucket b is [bend[b], bend[b + 1]).
-                    // During the count pass the 32-bit bucket counters live in the
-                    // pair area, which the fill pass overwrites afterwards.
+                    // bucket b is [bend[b], bend[b + 1]).  ends = bend + 1 starts on
+                    // a 16-byte boundary, so that one thread's eight ends are one
+                    // 128-bit word.
+                    // One table of 8192 buckets (fast): the count pass counts into
+                    // ends (16-bit, packed atomics) and the scan turns the counts
+                    // into starts in place.  Otherwise the 32-bit counters live in
+                    // the pair area during the count pass (the fill pass overwrites
+                    // them) and the scan's starts are copied into ends.
                     uint2 *tkv = reinterpret_cast<uint2 *>(lds);   // kTableCap + kScan slots
                     uint32_t *bcnt = lds;                           // nb counters (count pass only)
-                    uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan));   // kBuckets + 1
-                    uint32_t *bend_w = lds + 2 * (kTableCap + kScan);   // the same, as words (two ends each)
+                    uint16_t *bend = reinterpret_cast<uint16_t *>(lds + 2 * (kTableCap + kScan)) + 7;   // kBuckets + 1
+                    uint16_t *ends = bend + 1;
+                    uint32_t *ends_w = reinterpret_cast<uint32_t *>(ends);   // two ends a word
                     // A partition larger than one table is split by high hash bits
                     // into Q sub-tables of kSubFill tuples on average (the slack up
                     // to kTableCap covers the spread of their sizes).  One that still
@@ -1485,6 +1496,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                     uint32_t Q = nr <= kTableCap ? 1u : (nr + kSubFill - 1) / kSubFill;
                     uint32_t splits = 0;
                     const uint32_t lb = min(I.table_log2, kBucketBits), nb = 1u << lb;   // buckets
+                    const bool fast = nb == 8 * NT && nr <= kTableCap;   // (then Q == 1)
                     constexpr uint32_t RPT = (kTableCap + NT - 1) / NT;
                     // the bucket of a key: multiply-shift hashing, or (sort-merge)
                     // its place in the partition's key range [kmin, kmax] cut into
@@ -1504,7 +1516,9 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         {
                             uint32_t i0 = threadIdx.x;   // opaque: see the bucket ends below
                             asm volatile("" : "+v"(i0));
-                            for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
+                            if (fast) reinterpret_cast<uint4 *>(ends)[i0] = make_uint4(0u, 0u, 0u, 0u);
+                            else
+                                for (uint32_t i = i0; i < nb; i += NT) bcnt[i] = 0;
                         }
                         if (RANGE && threadIdx.x == 0) {
                             sh.kmin = 0xFFFFFFFFu;
@@ -1519,9 +1533,14 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                         for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
                         if (Q == 1) {
+                            // opaque lane: the positions are formed here, not hoisted
+                            // to the item's start (where one spills, and its reload
+                            // waits, in order, for the previous item's pair stores)
+                            uint32_t l0 = threadIdx.x;
+                            asm volatile("" : "+v"(l0));
 #pragma unroll
                             for (uint32_t u = 0; u < RPT; u++) {
-                                const uint32_t i = rb + u * NT + threadIdx.x;
+                                const uint32_t i = rb + u * NT + l0;
                                 if (i < re) r[u] = gld(I.Rp, rpos(i));
                             }
                             if constexpr (RANGE) {
@@ -1552,9 +1571,19 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                                                                 0xFFFFFFFFull)
                                                       : 0u;
                             }
+                            if (fast) {
+                                // a packed 16-bit counter never carries: counts <= nr < 2^16
 #pragma unroll
-                            for (uint32_t u = 0; u < RPT; u++)
-                                if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bkt(r[u].x)], 1u);
+                                for (uint32_t u = 0; u < RPT; u++) {
+                                    if (r[u].x == kEmptyKey) continue;
+                                    const uint32_t b = bkt(r[u].x);
+                                    atomicAdd(&ends_w[b >> 1], 1u << ((b & 1u) * 16u));
+                                }
+                            } else {
+#pragma unroll
+                                for (uint32_t u = 0; u < RPT; u++)
+                                    if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bkt(r[u].x)], 1u);
+                            }
                         } else {
                             for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
                                 uint2 c[4];
@@ -1571,7 +1600,42 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             }
                         }
                         __syncthreads();
-                        const uint32_t ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                        uint32_t ntab;
+                        if (fast) {
+                            // thread t: buckets 8t .. 8t + 7, one 128-bit word of
+                            // counts, scanned in registers; the wave prefix by DPP,
+                            // the waves' totals through wtmp (one barrier)
+                            uint32_t t0 = threadIdx.x;   // opaque: addresses formed here
+                            asm volatile("" : "+v"(t0));
+                            uint4 *ev = reinterpret_cast<uint4 *>(ends) + t0;
+                            const uint4 c = *ev;
+                            const uint32_t e[8] = {c.x & 0xFFFFu, c.x >> 16, c.y & 0xFFFFu, c.y >> 16,
+                                                   c.z & 0xFFFFu, c.z >> 16, c.w & 0xFFFFu, c.w >> 16};
+                            uint32_t sum = 0;
+#pragma unroll
+                            for (int k = 0; k < 8; k++) sum += e[k];
+                            const uint32_t x = wave_incl_sum(sum);
+                            const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+                            if (lane_id() == kWave - 1) wtmp[wid] = x;
+                            __syncthreads();
+                            uint32_t pre = 0, tot = 0;
+#pragma unroll
+                            for (uint32_t k = 0; k < NT / kWave; k++) {
+                                const uint32_t v = wtmp[k];
+                                tot += v;
+                                pre += k < wid ? v : 0u;
+                            }
+                            uint32_t run = pre + x - sum, o[4];
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                o[k] = run | ((run + e[2 * k]) << 16);
+                                run += e[2 * k] + e[2 * k + 1];
+                            }
+                            *ev = make_uint4(o[0], o[1], o[2], o[3]);   // ends[b] = start of bucket b
+                            ntab = __builtin_amdgcn_readfirstlane(tot);
+                        } else {
+                            ntab = block_exclusive_scan(bcnt, nb, wtmp);   // bcnt[b] = start of bucket b
+                        }
                         if (ntab > kTableCap) {   // uniform: ntab is the same in every thread
                             if (!advanced) {   // the queue and reservation atomics, as below
                                 uint32_t nx = 0, rbx = 0;
@@ -1601,15 +1665,15 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             continue;
                         }
                         // 16-bit fill cursors (starts < 2^16: ntab <= kTableCap); the
-                        // cursor of bucket b is bend[b + 1]
-                        {
+                        // cursor of bucket b is ends[b] = bend[b + 1]
+                        if (!fast) {
                             // the lane index is made opaque so that the addresses are
                             // formed here: hoisted out of the item loop they spill,
                             // and the reload waits (in order) for the previous item's
                             // pair stores
                             uint32_t i0 = threadIdx.x;
                             asm volatile("" : "+v"(i0));
-                            for (uint32_t i = i0; i < nb; i += NT) bend[i + 1] = (uint16_t)bcnt[i];
+                            for (uint32_t i = i0; i < nb; i += NT) ends[i] = (uint16_t)bcnt[i];
                         }
                         if (threadIdx.x == 0) bend[0] = 0;
                         __syncthreads();
@@ -1619,8 +1683,8 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                             for (uint32_t u = 0; u < RPT; u++) {
                                 if (r[u].x == kEmptyKey) continue;
-                                const uint32_t c = bkt(r[u].x) + 1u, sh16 = (c & 1u) * 16u;
-                                const uint32_t pos = (atomicAdd(&bend_w[c >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                                const uint32_t b = bkt(r[u].x), sh16 = (b & 1u) * 16u;
+                                const uint32_t pos = (atomicAdd(&ends_w[b >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
                                 tkv[pos] = r[u];
                             }
                         } else {
@@ -1634,8 +1698,8 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
                                     if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
-                                    const uint32_t cb = bucket_of(c[u].x, lb) + 1u, sh16 = (cb & 1u) * 16u;
-                                    const uint32_t pos = (atomicAdd(&bend_w[cb >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+                                    const uint32_t b = bucket_of(c[u].x, lb), sh16 = (b & 1u) * 16u;
+                                    const uint32_t pos = (atomicAdd(&ends_w[b >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
                                     tkv[pos] = c[u];
                                 }
                             }
@@ -2083,7 +2147,7 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 
 size_t lds_join()
 {
-    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 2) * 2;
+    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 8) * 2;   // ends 16-byte aligned: 7 + 1 + kBuckets halves
     size_t smj = ((size_t)kSortPad + kSortTail) * 8;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
