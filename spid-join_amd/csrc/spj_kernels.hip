@@ -1532,12 +1532,27 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         uint2 r[RPT];
 #pragma unroll
                         for (uint32_t u = 0; u < RPT; u++) r[u] = make_uint2(kEmptyKey, 0u);   // defined on every path
+                        // One table of 8192 buckets: the queue and reservation
+                        // atomics and the load of the next item's words are issued
+                        // before R's loads, so that the count's wait for R (vector
+                        // memory completes in issue order) covers them, instead of a
+                        // wait of their own at the probe's start.
+                        const bool early = fast && !advanced;
+                        uint32_t e_nx = 0, e_rb = 0, e_pre = 0, e_it2 = total_items;
                         if (Q == 1) {
                             // opaque lane: the positions are formed here, not hoisted
                             // to the item's start (where one spills, and its reload
                             // waits, in order, for the previous item's pair stores)
                             uint32_t l0 = threadIdx.x;
                             asm volatile("" : "+v"(l0));
+                            if (early) {
+                                if (threadIdx.x < kItemWords) e_it2 = sh.item2;
+                                if (e_it2 < total_items) e_pre = gld32(reinterpret_cast<const uint32_t *>(items + e_it2), l0);
+                                if (threadIdx.x == 0) {
+                                    e_nx = atomicAdd(&ctrl[1], 1u);
+                                    e_rb = atomicAdd(&out_lo[w.inst], w.se - w.sb);
+                                }
+                            }
 #pragma unroll
                             for (uint32_t u = 0; u < RPT; u++) {
                                 const uint32_t i = rb + u * NT + l0;
@@ -1583,6 +1598,18 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
 #pragma unroll
                                 for (uint32_t u = 0; u < RPT; u++)
                                     if (r[u].x != kEmptyKey) atomicAdd(&bcnt[bkt(r[u].x)], 1u);
+                            }
+                            if (early) {
+                                // the next item's words and index, this item's
+                                // reservation (read after the barriers below)
+                                if (e_it2 < total_items) nw[threadIdx.x] = e_pre;
+                                if (threadIdx.x == 0) {
+                                    if (e_it2 < total_items) nw_item = e_it2;
+                                    sh.item = sh.item2;
+                                    sh.item2 = e_nx;
+                                    sh.base = e_rb;
+                                }
+                                advanced = true;
                             }
                         } else {
                             for (uint32_t i0 = rb; i0 < re; i0 += 4 * NT) {
@@ -1706,6 +1733,21 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         }
                         if (threadIdx.x == 0) sh.chunk = NT / kWave;   // the probe's first chunks go to waves 0..15
                         __syncthreads();   // bend[b + 1] = end of bucket b
+                        constexpr uint32_t CH = kWave * IPT;
+                        const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
+                        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+                        const uint32_t ln = lane_id();
+                        // (issued before the duplicate check, which hides their latency)
+                        uint2 nv[IPT];
+                        {
+                            // the lane index is made opaque so that the first chunk's
+                            // positions are mapped here, not hoisted to the item's
+                            // start (where they would spill, and their reload wait,
+                            // in order, behind every store in flight)
+                            uint32_t lane = ln;
+                            asm volatile("" : "+v"(lane));
+                            load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
+                        }
                         if constexpr (RANGE) {
                             // sort each key-range bucket in place (its entries are
                             // contiguous): four or fewer in registers by a network,
@@ -1837,20 +1879,6 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         // a counter, so that waves that finish early take over the
                         // rest instead of waiting at the item's end for slow ones.
                         // The next chunk's loads are in flight while one is probed.
-                        constexpr uint32_t CH = kWave * IPT;
-                        const uint32_t nch = (w.se - w.sb + CH - 1) / CH;
-                        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
-                        const uint32_t ln = lane_id();
-                        uint2 nv[IPT];
-                        {
-                            // the lane index is made opaque so that the first chunk's
-                            // positions are mapped here, not hoisted to the item's
-                            // start (where they would spill, and their reload wait,
-                            // in order, behind every store in flight)
-                            uint32_t lane = ln;
-                            asm volatile("" : "+v"(lane));
-                            load_chunk<IPT>(nv, I.Sp, spos, w.sb + wv * CH, w.se, lane);
-                        }
                         uint32_t cg = 0;   // lane 0: the chunk after this wave's first (none without a first)
                         if (ln == 0 && wv < nch) cg = atomicAdd(&sh.chunk, 1u);
                         uint32_t nx = 0, rbx = 0, pre = 0;
