@@ -116,7 +116,7 @@ def roofline(res, workload):
     b_alg = 24 * (nR + nS) + 8 * res["pairs"]
     return name, {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                   "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(name, workload)}, {
-        k: {"ms": round(v[0], 4), "alg_bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)}
+        k: {"ms": round(v[0], 4), "alg_bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else 0.0}
         for k, v in kern.items()}, b_alg
 
 

@@ -27,14 +27,14 @@ namespace spj {
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kEmptyKey = 0u;          // key 0 = empty / skipped (glb_partition_count.c:192)
-constexpr uint32_t kHeavySlots = 4096;      // heavy (key -> R rid) table, LDS resident
+constexpr uint32_t kHeavySlots = 2048;      // heavy (key -> R rid) table, LDS resident
 // One instance's heavy table in device memory (and in LDS, in the same
 // order): kHeavySlots keys (0 = empty), kHeavySlots R rids, then a presence bitmap (bit s set: slot s's key has an
 // R tuple and its rid is valid).  Presence is kept apart from the rid because
 // every 32-bit value is a legal rid (kv_pair_t.rvalue is a full uint32).
 constexpr uint32_t kHeavyPresWords = kHeavySlots / 32;
 constexpr uint32_t kHeavyWords = 2 * kHeavySlots + kHeavyPresWords;
-constexpr uint32_t kHeavyMaxKeys = 1024;    // at most a quarter full
+constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
 constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
 constexpr uint32_t kScatterGroups = 8;      // scatter groups (one per XCD) per relation

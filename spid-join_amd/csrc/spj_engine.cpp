@@ -133,7 +133,11 @@ Context::Context()
     const char *g = getenv("SPJ_GRAPHS");   // SPJ_GRAPHS=0: never replay launch sequences as graphs
     graphs_ = !(g && g[0] == '0');
     (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
-    for (auto &e : ev_) (void)hipEventCreate(&e);
+    // phase-timing events record without a system-scope fence: a fenced
+    // record writes back and invalidates the caches (several microseconds
+    // each, and the next kernel starts cold); only the join's end needs to
+    // be visible to the host, and the result copy's synchronisation does that
+    for (auto &e : ev_) (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     (void)hipEventCreateWithFlags(&desc_ev_, hipEventDisableTiming);
     // page-locked staging is slow to allocate: take a size that covers
     // most batches now, not in the first join (a latency outlier)
@@ -598,11 +602,15 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
 
     // ---- partition: histogram (counted tiles), scan, scatter
     SPJ_TRY(launch_hist(d_insts, d_tiles + hist_first, n_counted, counts_.as<uint32_t>(), max_bits_, st));
-    if (timed) SPJ_TRY(hipEventRecord(ev_[2], st));
+    // phases that launch nothing get no event (an event between two others
+    // still costs the device a few microseconds)
+    hist_timed_ = timed && n_counted != 0;
+    scan_timed_ = timed && !h_segs_.empty();
+    if (hist_timed_) SPJ_TRY(hipEventRecord(ev_[2], st));
     SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), d_ad, (uint32_t)h_ad_.size(), d_b, (uint32_t)h_b_.size(),
                         counts_.as<uint32_t>(), part_.as<uint32_t>(),
                         pstart_.as<uint32_t>(), max_bits_, st));
-    if (timed) SPJ_TRY(hipEventRecord(ev_[3], st));
+    if (scan_timed_) SPJ_TRY(hipEventRecord(ev_[3], st));
     // one persistent 1024-thread workgroup per CU (LDS-bound); R first
     // when its tiles record heavy rids for the S tiles
     const uint32_t sgrid = (uint32_t)std::max(1, device_info().cus);
@@ -709,8 +717,15 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         SPJ_TRY(compact(i, lo, hi, fill, holes));
     }
     if (timed) {
+        // boundaries of the phases; a phase that launched nothing ends where it began
+        hipEvent_t b[7] = {ev_[0], ev_[1], ev_[2], ev_[3], ev_[4], ev_[5], ev_[6]};
+        if (!hist_timed_) b[2] = b[1];
+        if (!scan_timed_) b[3] = b[2];
         float ms[6];
-        for (int i = 0; i < 6; i++) SPJ_TRY(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+        for (int i = 0; i < 6; i++) {
+            ms[i] = 0.f;
+            if (b[i] != b[i + 1]) SPJ_TRY(hipEventElapsedTime(&ms[i], b[i], b[i + 1]));
+        }
         timed->heavy = ms[0];
         timed->hist = ms[1];
         timed->scan = ms[2];
@@ -719,7 +734,7 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         timed->join = ms[5];
         SPJ_TRY(hipEventElapsedTime(&timed->total, ev_[0], ev_[6]));
         timed->scatter_r = -1;
-        if (split_timed_) SPJ_TRY(hipEventElapsedTime(&timed->scatter_r, ev_[3], ev_[7]));
+        if (split_timed_) SPJ_TRY(hipEventElapsedTime(&timed->scatter_r, b[3], ev_[7]));
         if (n) {
             timed->heavy_keys = h_ctrl_[ctrl_at(n, kCtrlHeavy, 0)];
             const InstDesc &d0 = h_insts_[0];

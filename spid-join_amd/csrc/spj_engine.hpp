@@ -126,6 +126,8 @@ private:
     uint64_t graph_captures_ = 0, graph_replays_ = 0;
     hipEvent_t ev_[8] = {};   // [0, 6]: phase boundaries; [7]: end of R's own scatter launch
     bool split_timed_ = false;  // the last timed enqueue recorded ev_[7]
+    bool hist_timed_ = false;   // ... ev_[2] (it launched a histogram pass)
+    bool scan_timed_ = false;   // ... ev_[3] (it launched a scan)
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
     DevBuf desc_, counts_, part_, pstart_;

@@ -78,6 +78,25 @@ __device__ __forceinline__ uint32_t sfh(uint32_t key)
     return h;
 }
 
+// The low bits of sfh(key) alone, for digits of up to 19 bits: the step
+// h ^= h << 25 changes only bits 25 and up, and the final h += h >> 6 brings
+// bits up to 24 down no further than bit 18, so bits 0..18 of the result do
+// not depend on it.
+__device__ __forceinline__ uint32_t sfh_low(uint32_t key)
+{
+    uint32_t h = 4u + (key & 0xFFFFu);
+    uint32_t tmp = ((key >> 16) << 11) ^ h;
+    h = (h << 16) ^ tmp;
+    h += h >> 11;
+    h ^= h << 3;
+    h += h >> 5;
+    h ^= h << 4;
+    h += h >> 17;
+    h += h >> 6;
+    return h;
+}
+static_assert(kMaxBits <= 19, "sfh_low gives digits of up to 19 bits");
+
 // Bucket of a key in the join's LDS table: Fibonacci (multiply-shift)
 // hashing of the key.  Partitions are chosen by SuperFastHash bits, so the
 // two are independent.
@@ -88,8 +107,10 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_bucket
 
 // Heavy table: kHeavySlots / 4 buckets of 4 slots; a key lives in one of
 // two buckets (2-choice), so a lookup is two independent 16-byte LDS reads
-// and never loops.
-constexpr uint32_t kHeavyBucketBits = 10;   // 1024 buckets x 4 slots = kHeavySlots
+// and never loops.  At most kHeavyMaxKeys keys (half full): placing 1024
+// keys hottest first into 512 such buckets leaves about one key in 100
+// selections without a slot (it stays an ordinary key).
+constexpr uint32_t kHeavyBucketBits = 9;    // 512 buckets x 4 slots = kHeavySlots
 static_assert(kHeavySlots == (4u << kHeavyBucketBits), "heavy table: 4-slot buckets");
 __device__ __forceinline__ uint32_t heavy_b1(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - kHeavyBucketBits); }
 __device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
@@ -317,7 +338,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
             if (mode == 2 && hs[j] >= 0) continue;   // heavy S tuple: no partition
-            atomicAdd(&hist[sfh(v[j].x) & mask], 1u);
+            atomicAdd(&hist[sfh_low(v[j].x) & mask], 1u);
             if (mode == 1 && hs[j] >= 0) heavy_record_rid(I, (uint32_t)hs[j], v[j].y);
         }
     }
@@ -450,7 +471,7 @@ __device__ __forceinline__ void scatter_write_out(const uint2 *stage, const uint
     asm volatile("" : "+v"(i0));
     for (uint32_t i = i0; i < hstart; i += kScatterNT) {
         const uint2 t = stage[i];
-        gst(outp, bs[sfh(t.x) & mask] + i, t);
+        gst(outp, bs[sfh_low(t.x) & mask] + i, t);
     }
     for (uint32_t i = hstart + i0; i < total; i += kScatterNT) {
         const uint32_t oi = hb + (i - hstart);
@@ -478,9 +499,12 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t *hk = lds;                                             // kHeavySlots (16-byte aligned)
     uint32_t *hr = hk + kHeavySlots;                                // kHeavySlots rids + presence bits
     uint2 *stage = reinterpret_cast<uint2 *>(hr + kHeavySlots + kHeavyPresWords);   // ROUND tuples
-    uint32_t *lst = reinterpret_cast<uint32_t *>(stage + ROUND);    // FM + 2: counts -> starts
-    uint32_t *bs = lst;                                             // after staging: write base - start
-    uint32_t *wtmp = lst + FM + 2;                                  // NT/64 + 1
+    // two rounds' digit counters (FM + 2 each), alternating: round r counts
+    // into one (counts -> starts -> write base - start) while the other
+    // still holds the bases that round r - 1's write-out reads
+    uint32_t *lsa = reinterpret_cast<uint32_t *>(stage + ROUND);
+    uint32_t *lsb = lsa + FM + 2;
+    uint32_t *wtmp = lsb + FM + 2;                                  // NT/64 + 1
     uint32_t *misc = wtmp + NT / kWave + 1;                         // [0] heavy output base, [1] queue
 
     // Tiles come from the XCD's queue, taken two ahead: t is being
@@ -491,7 +515,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         misc[1] = atomicAdd(q, 1u);
         misc[2] = atomicAdd(q, 1u);
     }
-    for (uint32_t i = threadIdx.x; i < FM + 2; i += NT) lst[i] = 0;
+    for (uint32_t i = threadIdx.x; i < 2 * (FM + 2); i += NT) lsa[i] = 0;
     __syncthreads();
     uint32_t t = x + kScatterGroups * misc[0], tnext = x + kScatterGroups * misc[1];
     if (t >= te) return;   // whole workgroup
@@ -502,11 +526,22 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t o = td.begin;
     uint32_t heavy_inst = DROP;   // instance whose heavy table is in LDS
     // software pipeline: the next round's tuples (possibly of the next tile)
-    // are loaded while this round is ranked, staged and written
+    // are loaded while this round is scanned, staged and written
     uint2 nv[IPT];
     load_tile_round<IPT>(nv, insts, td, o);
     __builtin_amdgcn_s_waitcnt(0);   // entering the loop with no loads in flight (see the wait below)
+    bool par = false;   // which counter array this round uses
+    // No barrier between a round's write-out and the next round's ranking:
+    // a wave that has issued its stores goes on to rank while slower waves
+    // still write, and the ranking barrier also marks the previous round's
+    // write-out as done.  The next round's loads are issued after the rank
+    // atomics, not at the round's start: issued while the other waves' stores
+    // were still queued, they made the scatter 7% (Zipf 0) to 19% (Zipf 1.5)
+    // slower on MI355X.
     for (;;) {
+        uint32_t *const lst = par ? lsb : lsa;    // this round's counters
+        uint32_t *const lold = par ? lsa : lsb;   // the previous round's bases
+        uint32_t *const bs = lst;                 // after staging: write base - start
         uint32_t tn = t, on = o + ROUND;
         TileDesc tdn = td;
         if (on >= td.end) {
@@ -520,7 +555,6 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         uint2 v[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; j++) v[j] = nv[j];
-        if (more) load_tile_round<IPT>(nv, insts, tdn, on);
 
         const InstDesc &I = insts[td.inst];
         const uint32_t F = 1u << I.bits, mask = F - 1u;
@@ -530,8 +564,9 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         // a later launch
         const bool rec = (I.heavy != 0) && (td.rel == 0) && (td.cap != 0) && heavy_keys_of(I) != 0;
         const uint32_t rn = min(ROUND, td.end - o);
-        __syncthreads();   // the previous round's write-out and clearing are done
         if ((heavy || rec) && heavy_inst != td.inst) {
+            // the last lookups in the old table were before the previous
+            // round's ranking barrier
             load_heavy(I, hk, kHeavyWords);
             heavy_inst = td.inst;
             __syncthreads();
@@ -561,7 +596,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 const uint32_t li = j * NT + threadIdx.x;
                 dg[j] = DROP;
                 if (li < rn && v[j].x != kEmptyKey) {
-                    if (hs[j] < 0) dg[j] = sfh(v[j].x) & mask;
+                    if (hs[j] < 0) dg[j] = sfh_low(v[j].x) & mask;
                     else { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
                 }
                 htot += (uint32_t)__popcll(__ballot(dg[j] == F));
@@ -594,13 +629,21 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
-                dg[j] = (li < rn && v[j].x != kEmptyKey) ? (sfh(v[j].x) & mask) : DROP;
+                dg[j] = (li < rn && v[j].x != kEmptyKey) ? (sfh_low(v[j].x) & mask) : DROP;
             }
         }
 #pragma unroll
         for (int j = 0; j < IPT; j++)
             if (dg[j] != DROP && dg[j] != F) rk[j] = atomicAdd(&lst[dg[j]], 1u);
-        __syncthreads();
+        if (more) load_tile_round<IPT>(nv, insts, tdn, on);
+        __syncthreads();   // ranks counted; the previous round's write-out is done
+        {
+            // the previous round's counters, for the round after this one
+            // (its ranking comes after this round's last barrier)
+            uint32_t i0 = threadIdx.x;   // opaque, as in the write-out
+            asm volatile("" : "+v"(i0));
+            for (uint32_t i = i0; i < FM + 2; i += NT) lold[i] = 0;
+        }
         // reserve each digit's run at the group cursor; the returned bases
         // are only needed after the scan and staging, which hide the latency
         uint32_t *cur = counts + td.cnt_off;
@@ -648,11 +691,10 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 stage[p] = v[j];
             }
         }
-        // One unconditional wait for all vector memory operations: the
-        // reservations, and with them (completion is in issue order) the
-        // next round's loads.  The compiler then knows those loads are done
-        // at the next round's start and does not wait there -- a wait that
-        // would also cover this round's stores, issued after them.
+        // One unconditional wait for all vector memory operations: the next
+        // round's loads and the reservations.  The compiler then knows those
+        // loads are done at the next round's start and does not wait there --
+        // a wait that would also cover this round's stores, issued after them.
         __builtin_amdgcn_s_waitcnt(0);
         if (qwant) {   // thread 0; the atomic completed with the wait above
             misc[1] = qv;
@@ -681,13 +723,8 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         __syncthreads();
         scatter_write_out(stage, bs, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
                           &err[td.inst]);
-        __syncthreads();   // the write-out has read the bases
-        {
-            uint32_t i0 = threadIdx.x;   // opaque, as in the write-out
-            asm volatile("" : "+v"(i0));
-            for (uint32_t i = i0; i < F + 2; i += NT) lst[i] = 0;
-        }
         if (!more) break;
+        par = !par;
         if (tn != t) {   // next tile: the one after it was taken a round ago at least
             tnext = x + kScatterGroups * misc[1];
             qwant = threadIdx.x == 0;
@@ -733,7 +770,9 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
             const uint32_t m = (lo + hi) >> 1;
             if (jobs[m].first_block[phase] <= vb) lo = m; else hi = m;
         }
-        const HeavyJob job = jobs[lo];
+        // the job's fields are read where they lie (a local copy indexed by
+        // phase would live in scratch memory)
+        const HeavyJob &job = jobs[lo];
         const InstDesc &I = insts[job.inst];
         uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
         uint32_t *found = cand_cnt + kCandSlots;
@@ -743,19 +782,21 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
         const uint32_t k = phase == 0 ? l * kPreSample : (l / (kPreSample - 1)) * kPreSample + l % (kPreSample - 1) + 1;
         if (k >= job.nblocks) continue;
         if (phase == 1 && *found == 0) continue;   // no candidate in the pre-sample: no skew worth the rest
-        for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
-        __syncthreads();
+        const uint2 *S = I.S;   // read once: reloaded per key, each key's load would wait for it
         const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
         const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
         // all of the block's keys are loaded first (kSampleBlock / 1024 per
-        // thread), so the loads overlap instead of waiting one by one
+        // thread), so the loads overlap instead of waiting one by one; the
+        // table is cleared while they are in flight
         constexpr uint32_t KPT = kSampleBlock / 1024;
         uint32_t keys[KPT];
 #pragma unroll
         for (uint32_t u = 0; u < KPT; u++) {
             const uint64_t i = b + u * 1024 + threadIdx.x;
-            keys[u] = (i < e) ? gld(I.S, i).x : kEmptyKey;
+            keys[u] = (i < e) ? gld(S, i).x : kEmptyKey;
         }
+        for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) { sk[i] = 0; sc[i] = 0; }
+        __syncthreads();
 #pragma unroll
         for (uint32_t u = 0; u < KPT; u++) {
             const uint32_t key = keys[u];
@@ -979,16 +1020,25 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
 // partitioned S and R tuples.  Grid (instances, partition chunks of kItemsNT).
 // ---------------------------------------------------------------------------
 // Partition p's position map from its cells' fills (clamped to the
-// capacity); returns the partition's tuple count.
-__device__ __forceinline__ uint32_t cell_map(const CellGroup *cg, const uint32_t *counts, uint32_t p, SMap &m)
+// capacity); returns the partition's tuple count.  The group descriptors
+// and all fills are loaded before any is used, so the loads overlap.
+__device__ __forceinline__ uint32_t cell_map(const CellGroup *__restrict__ cg, const uint32_t *__restrict__ counts,
+                                             uint32_t p, SMap &m)
 {
+    uint32_t cap[kScatterGroups], cb[kScatterGroups], cur[kScatterGroups];
+#pragma unroll
+    for (uint32_t g = 0; g < kScatterGroups; g++) {
+        cap[g] = cg[g].cap;
+        cb[g] = cg[g].cell0 + p * cap[g];
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < kScatterGroups; g++) cur[g] = cap[g] ? counts[cg[g].cnt_off + p] : cb[g];
     uint32_t n = 0;
 #pragma unroll
     for (uint32_t g = 0; g < kScatterGroups; g++) {
-        const uint32_t cap = cg[g].cap, cb = cg[g].cell0 + p * cap;
-        const uint32_t fill = cap ? min(counts[cg[g].cnt_off + p] - cb, cap) : 0u;
+        const uint32_t fill = min(cur[g] - cb[g], cap[g]);
         m.V[g] = n;
-        m.D[g] = cb - n;
+        m.D[g] = cb[g] - n;
         n += fill;
     }
     return n;
@@ -1063,16 +1113,11 @@ __global__ __launch_bounds__(kItemsNT) void k_make_items(const InstDesc *__restr
             w.rmap = rm;
             items[o + k] = w;
         }
-        __syncthreads();
-        cnt[threadIdx.x] = ns;
-        __syncthreads();
-        const uint32_t all_s = block_exclusive_scan(cnt, kItemsNT, wtmp);
-        cnt[threadIdx.x] = nr;
-        __syncthreads();
-        const uint32_t all_r = block_exclusive_scan(cnt, kItemsNT, wtmp);
-        if (threadIdx.x == 0) {
-            if (all_s) atomicAdd(&spart[inst], all_s);
-            if (all_r) atomicAdd(&rpart[inst], all_r);
+        // partitioned tuples of the instance: one atomic per wave and relation
+        const uint32_t ws = wave_incl_sum(ns), wr = wave_incl_sum(nr);
+        if (lane_id() == kWave - 1) {
+            if (ws) atomicAdd(&spart[inst], ws);
+            if (wr) atomicAdd(&rpart[inst], wr);
         }
     }
 }
@@ -2169,7 +2214,7 @@ size_t lds_hist(uint32_t max_bits)
 size_t lds_scatter(uint32_t max_bits, int ipt)
 {
     const size_t F = size_t(1) << max_bits, R = (size_t)kScatterNT * (size_t)ipt;
-    return (size_t)kHeavyWords * 4 + R * 8 + (F + 2) * 4 +
+    return (size_t)kHeavyWords * 4 + R * 8 + 2 * (F + 2) * 4 +
            (kScatterNT / kWave + 1) * 4 + 16;
 }
 
