@@ -68,6 +68,13 @@ typedef struct {
                                  overflows); AUTO = cells for a hash or sort-merge relation of >= 2^20 tuples
                                  (for S only with heavy-key handling on) */
     int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
+    int32_t radix_passes;     /* radix partition passes, 1 or 2 (single-instance calls only).  Two:
+                                 the first pass partitions R and S by radix_bits of
+                                 SuperFastHash into contiguous partitions (heavy keys joined in its
+                                 S pass), the second splits every partition by the next bits
+                                 (chosen per partition so R partitions fit one LDS table) and
+                                 joins it.  AUTO = 2 when one pass of 12 bits would leave R
+                                 partitions of 4 or more LDS tables (|R| > 3 * 2^26), else 1 */
 } spj_config_t;
 
 /* Per-phase device time of the last call (HIP events on its stream) and, for
@@ -79,6 +86,7 @@ typedef struct {
     uint64_t r_partitioned, s_partitioned;
     uint32_t s_cells;   /* 1: S went to fixed-capacity cells (no histogram pass), 0: counted */
     uint32_t r_cells;   /* the same for R */
+    uint32_t passes;    /* radix partition passes the call ran (hist, scan and scatter sum both) */
 } spj_phase_times_t;
 
 void spj_config_default(spj_config_t *cfg);

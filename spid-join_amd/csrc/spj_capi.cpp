@@ -60,6 +60,7 @@ Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_
     if (cfg->items_s_max > 0) r.smax = (uint32_t)cfg->items_s_max;
     if (cfg->part_layout >= 0) r.cells = (uint32_t)cfg->part_layout;
     if (cfg->cell_capacity_pct >= 0) r.cell_pct = (uint32_t)cfg->cell_capacity_pct;
+    if (cfg->radix_passes >= 0) r.passes = (uint32_t)cfg->radix_passes;
     return r;
 }
 
@@ -95,7 +96,7 @@ static bool auto_no_partition(const spj_config_t *cfg, uint64_t r, uint64_t s, u
     if (!cfg) return true;
     return cfg->algorithm < 0 && cfg->radix_bits < 0 && cfg->table_log2 < 0 && cfg->heavy < 0 &&
            cfg->heavy_min_count < 0 && cfg->scatter_ipt < 0 && cfg->tile_tuples <= 0 && cfg->items_s_max <= 0 &&
-           cfg->part_layout < 0 && cfg->cell_capacity_pct < 0;
+           cfg->part_layout < 0 && cfg->cell_capacity_pct < 0 && cfg->radix_passes < 0;
 }
 
 // AUTO's no-partition choice for one instance: the rule above, unless the
@@ -121,6 +122,9 @@ int check_resolved(const Resolved &r, uint64_t nR)
     if (r.smax < 512) return fail(SPJ_EINVAL, "items_s_max %u < 512", r.smax);
     if (r.cells > 2) return fail(SPJ_EINVAL, "part_layout %u (0 counted, 1 cells)", r.cells);
     if (r.cell_pct > 200) return fail(SPJ_EINVAL, "cell_capacity_pct %u > 200", r.cell_pct);
+    if (r.passes < 1 || r.passes > 2) return fail(SPJ_EINVAL, "radix_passes %u (1 or 2)", r.passes);
+    if (r.passes == 2 && (r.alg == SPJ_ALG_NPHJ || r.bits == 0))
+        return fail(SPJ_EINVAL, "two radix passes need a partitioned join and radix_bits >= 1");
     return SPJ_OK;
 }
 
@@ -150,6 +154,7 @@ void fill_times(const PhaseTimes &t, spj_phase_times_t *o)
     o->s_partitioned = t.s_parted;
     o->s_cells = t.s_cells;
     o->r_cells = t.r_cells;
+    o->passes = t.passes;
 }
 
 int validate_sizes(uint64_t nR, uint64_t nS)
@@ -179,6 +184,7 @@ void spj_config_default(spj_config_t *cfg)
     cfg->items_s_max = -1;
     cfg->part_layout = -1;
     cfg->cell_capacity_pct = -1;
+    cfg->radix_passes = -1;
 }
 
 const char *spj_last_error(void) { return g_err.c_str(); }
@@ -287,6 +293,133 @@ int spj_dev_sync(void)
     return e == hipSuccess ? SPJ_OK : hip_fail(e, "hipDeviceSynchronize");
 }
 
+}  // extern "C"
+
+namespace spj {
+
+// The second pass's context: its buffers stay apart from the first pass's,
+// whose partitions it reads.  Used under the default context's lock.
+static Context &second_pass_context()
+{
+    static Context *c = nullptr;
+    if (!c) c = new Context();
+    return *c;
+}
+
+// A join in two radix passes.  The first partitions R and S by s0.cfg.bits
+// of SuperFastHash into contiguous partitions; heavy keys (if on) are
+// joined in its S pass, their pairs at the back of out.  The second is a
+// batch with one instance per first-pass partition: partition p of R and S,
+// split by the next bits (as many as make its R partitions fit one LDS
+// table) and joined, its pairs written from out + (start of S partition p).
+// Partitions are then packed: every instance's pairs, in partition order,
+// then the heavy pairs.  ref: the reference partitions twice too, globally
+// across DPUs (glb_partition_count.c) and locally inside each
+// (a2a_packetwise_srj_local_partition.c:323-350).
+static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &err, PhaseTimes *times)
+{
+    Context &c1 = default_context(), &c2 = second_pass_context();
+    InstSpec p1 = s0;
+    p1.partition_only = true;
+    p1.r_contiguous = true;
+    std::vector<uint32_t> cnt, errs;
+    PhaseTimes t1, t2;
+    hipError_t e = c1.run({p1}, cnt, errs, times ? &t1 : nullptr);
+    if (e != hipSuccess) return e;
+    const uint32_t heavy_pairs = cnt[0];
+    err = errs[0];
+    const uint2 *rp = nullptr, *sp = nullptr;
+    std::vector<uint32_t> pr, ps;
+    e = c1.partitions(0, &rp, &sp, pr, ps);
+    if (e != hipSuccess) return e;
+    const uint32_t F1 = 1u << s0.cfg.bits;
+    std::vector<InstSpec> sub;
+    std::vector<uint32_t> where;   // each instance's S partition start (its output offset)
+    for (uint32_t p = 0; p < F1; p++) {
+        const uint32_t nr = pr[p + 1] - pr[p], ns = ps[p + 1] - ps[p];
+        if (!nr || !ns) continue;   // no pairs: an S partition without R tuples matches nothing
+        InstSpec q;
+        q.R = rp + pr[p];
+        q.nR = nr;
+        q.S = sp + ps[p];
+        q.nS = ns;
+        q.out = s0.out + ps[p];
+        q.shift = s0.cfg.bits;
+        Resolved c = default_params(nr, ns);
+        c.bits = std::min(c.bits, kMaxDigitBits - s0.cfg.bits);
+        c.alg = s0.cfg.alg;
+        c.table_log2 = s0.cfg.table_log2;
+        c.ipt = s0.cfg.ipt;
+        c.smax = s0.cfg.smax;   // one value per batch
+        // cells (no histogram pass) unless the caller fixed the layout: the
+        // first pass took the keys hot enough to overflow one
+        c.cells = s0.cfg.cells == 2 ? 1u : s0.cfg.cells;
+        c.cell_pct = s0.cfg.cell_pct;
+        c.heavy = 0;   // taken by the first pass
+        c.passes = 1;
+        q.cfg = c;
+        sub.push_back(q);
+        where.push_back(ps[p]);
+    }
+    uint64_t total = heavy_pairs;
+    if (!sub.empty()) {
+        e = c2.run(sub, cnt, errs, times ? &t2 : nullptr);
+        if (e != hipSuccess) return e;
+        for (size_t k = 0; k < sub.size(); k++) {
+            err |= errs[k];
+            total += cnt[k];
+        }
+    }
+    // pack: instance k's pairs at where[k], then the heavy pairs from the back
+    std::vector<uint4> segs;
+    uint64_t dst = 0;
+    bool dense = true;
+    for (size_t k = 0; k < sub.size(); k++) {
+        if (!cnt[k]) continue;
+        dense = dense && where[k] == dst;
+        segs.push_back(make_uint4(where[k], (uint32_t)dst, cnt[k], 0));
+        dst += cnt[k];
+    }
+    if (heavy_pairs) {
+        const uint32_t hsrc = s0.nS - heavy_pairs;
+        dense = dense && hsrc == dst;
+        segs.push_back(make_uint4(hsrc, (uint32_t)dst, heavy_pairs, 0));
+    }
+    if (!dense && !err) {
+        e = c2.gather(s0.out, segs, total);
+        if (e != hipSuccess) return e;
+    }
+    count = (uint32_t)total;
+    if (times) {
+        *times = t1;
+        times->hist += t2.hist;
+        times->scan += t2.scan;
+        times->scatter += t2.scatter;
+        times->items = t2.items;
+        times->join = t2.join;
+        times->total = t1.total + t2.total;
+        times->r_parted = pr[F1];
+        times->s_parted = ps[F1];
+        times->passes = 2;
+    }
+    return hipSuccess;
+}
+
+hipError_t run_one(const InstSpec &s, uint32_t &count, uint32_t &err, PhaseTimes *times)
+{
+    if (s.cfg.passes == 2) return run_two_pass(s, count, err, times);
+    std::vector<uint32_t> counts, errs;
+    const hipError_t e = default_context().run({s}, counts, errs, times);
+    if (e != hipSuccess) return e;
+    count = counts[0];
+    err = errs[0];
+    return hipSuccess;
+}
+
+}  // namespace spj
+
+extern "C" {
+
 int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64_t nS, spj_pair_t *out,
                     uint64_t out_cap, uint64_t *out_count, const spj_config_t *cfg, spj_phase_times_t *times)
 {
@@ -305,13 +438,13 @@ int spj_join_device(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uin
     rc = check_resolved(s.cfg, nR);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(default_context_mutex());
-    std::vector<uint32_t> counts, errs;
     PhaseTimes t;
-    hipError_t e = default_context().run({s}, counts, errs, times ? &t : nullptr);
-    if (e != hipSuccess) return hip_fail(e, "join");
+    uint32_t count = 0, err = 0;
+    const hipError_t e = run_one(s, count, err, times ? &t : nullptr);
+    if (e != hipSuccess) return hip_fail(e, s.cfg.passes == 2 ? "join (two passes)" : "join");
     fill_times(t, times);
-    *out_count = counts[0];
-    return err_from_flags(errs[0], 0);
+    *out_count = count;
+    return err_from_flags(err, 0);
 }
 
 static_assert(SPJ_NOALLOC_MAX_R == kNoAllocMaxR, "the header states the kernels' limit");
@@ -427,6 +560,10 @@ int spj_join_batch_device(int n, const spj_tuple_t *const *R, const uint64_t *nR
         s.nS = (uint32_t)nS[i];
         s.cfg = resolve(cfg, nR[i], nS[i], -1.0);
         if (nphj && !tuner_has_entry((uint32_t)nR[i], (uint32_t)nS[i], -1.0)) s.cfg.alg = SPJ_ALG_NPHJ;
+        // a batch partitions in one pass (its instances' partitions larger
+        // than an LDS table are joined in sub-tables)
+        if (cfg && cfg->radix_passes == 2) return fail(SPJ_EINVAL, "two radix passes: single joins only");
+        s.cfg.passes = 1;
         rc = check_resolved(s.cfg, nR[i]);
         if (rc) return rc;
     }

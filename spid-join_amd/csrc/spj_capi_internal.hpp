@@ -16,6 +16,9 @@ int err_from_flags(uint32_t flags, size_t inst);
 Resolved resolve(const spj_config_t *cfg, uint64_t nR, uint64_t nS, double skew_hint);
 // r's fields are valid; with nR, also the algorithm's size limit for |R| = nR.
 int check_resolved(const Resolved &r, uint64_t nR = 0);
+// One instance on the default context (held locked by the caller), in one
+// radix pass or two (s.cfg.passes).
+hipError_t run_one(const InstSpec &s, uint32_t &count, uint32_t &err, PhaseTimes *times);
 // AUTO's choice of the no-partition join for a single call (see spj_capi.cpp).
 bool auto_no_partition_single(const spj_config_t *cfg, uint64_t nR, uint64_t nS);
 int validate_sizes(uint64_t nR, uint64_t nS);

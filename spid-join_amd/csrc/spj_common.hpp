@@ -36,6 +36,7 @@ constexpr uint32_t kHeavyPresWords = kHeavySlots / 32;
 constexpr uint32_t kHeavyWords = 2 * kHeavySlots + kHeavyPresWords;
 constexpr uint32_t kHeavyMaxKeys = 1024;    // at most half full
 constexpr uint32_t kMaxBits = 12;           // at most 4096 partitions per pass
+constexpr uint32_t kMaxDigitBits = 19;      // shift + bits of a pass: digits come from SuperFastHash bits 0..18
 constexpr uint32_t kScatterNT = 1024;       // scatter workgroup size
 constexpr uint32_t kScatterGroups = 8;      // scatter groups (one per XCD) per relation
 constexpr uint32_t kSinkTuples = kScatterNT * 12;   // largest scatter round
@@ -66,7 +67,11 @@ struct InstDesc {
     // offset of a kSinkTuples sink for runs that overflow a cell.
     uint32_t s_cells, s_sink;
     uint32_t r_cells, r_sink;
-    uint32_t pad_[2];
+    // The digit of a key is (SuperFastHash(key) >> shift) & (2^bits - 1):
+    // shift 0 for a one-pass partition and for the first of two passes; the
+    // second pass partitions each first-pass partition by the next bits.
+    uint32_t shift;
+    uint32_t pad_;
 };
 
 // Cell layout of one scatter group of one relation: the group's cursors

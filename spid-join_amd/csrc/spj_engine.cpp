@@ -125,6 +125,11 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     r.heavy = nS >= (1u << 20) ? 1u : 0u;
     r.heavy_min = 16;
     r.alg = 0;
+    // Two passes when one pass of kMaxBits would leave R partitions of four
+    // or more LDS tables: the join would read each partition's S that many
+    // times, against one more read and write of R and S for a second pass.
+    const uint64_t per = (uint64_t(nR) + (1u << kMaxBits) - 1) >> kMaxBits;
+    r.passes = per > 3ull * 16384 ? 2u : 1u;
     return r;
 }
 
@@ -188,7 +193,7 @@ static bool same_params(const Resolved &a, const Resolved &b)
 {
     return a.bits == b.bits && a.table_log2 == b.table_log2 && a.alg == b.alg && a.heavy == b.heavy &&
            a.heavy_min == b.heavy_min && a.ipt == b.ipt && a.tile == b.tile && a.smax == b.smax && a.cells == b.cells &&
-           a.cell_pct == b.cell_pct;
+           a.cell_pct == b.cell_pct && a.passes == b.passes;
 }
 
 static bool same_specs(const std::vector<InstSpec> &a, const std::vector<InstSpec> &b)
@@ -197,7 +202,8 @@ static bool same_specs(const std::vector<InstSpec> &a, const std::vector<InstSpe
     for (size_t i = 0; i < a.size(); i++) {
         const InstSpec &x = a[i], &y = b[i];
         if (x.R != y.R || x.nR != y.nR || x.S != y.S || x.nS != y.nS || x.out != y.out || x.Rp_pre != y.Rp_pre ||
-            x.pst_pre != y.pst_pre || x.r_contiguous != y.r_contiguous || !same_params(x.cfg, y.cfg))
+            x.pst_pre != y.pst_pre || x.r_contiguous != y.r_contiguous || x.shift != y.shift ||
+            x.partition_only != y.partition_only || !same_params(x.cfg, y.cfg))
             return false;
     }
     return true;
@@ -310,8 +316,13 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
                    (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= (1u << 20) && skew_safe));
         };
-        cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true);
-        cells[1][i] = !nphj && use_cells(s.nS, heavy_on);
+        // a first pass of two leaves contiguous partitions (the second
+        // pass's instances read them as plain relations)
+        if (s.partition_only != specs[0].partition_only || (s.partition_only && (nphj || pre)))
+            return hipErrorInvalidValue;
+        if (s.shift + c.bits > kMaxDigitBits) return hipErrorInvalidValue;
+        cells[0][i] = !pre && !nphj && !s.r_contiguous && !s.partition_only && use_cells(s.nR, true);
+        cells[1][i] = !nphj && !s.partition_only && use_cells(s.nS, heavy_on);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
@@ -319,6 +330,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         d.nR = s.nR;
         d.nS = s.nS;
         d.bits = c.bits;
+        d.shift = s.shift;
         d.heavy = heavy_on ? 1u : 0u;
         d.table_log2 = c.table_log2;
         d.alg = (c.alg == 1 && !nphj) ? 0u : c.alg;   // a resident table is partitioned: hash join
@@ -624,7 +636,18 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                            ctrl + 4 + kScatterGroups, max_bits_, max_ipt_, sgrid, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
-    // ---- join
+    // ---- join (none after a first pass of two)
+    partition_only_ = n && specs[0].partition_only;
+    if (partition_only_) {
+        if (timed) {
+            SPJ_TRY(hipEventRecord(ev_[5], st));
+            SPJ_TRY(hipEventRecord(ev_[6], st));
+        }
+        specs_n_ = n;
+        h_ns_.resize(n);
+        for (size_t i = 0; i < n; i++) h_ns_[i] = specs[i].nS;
+        return hipSuccess;
+    }
     SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), d_cells, counts_.as<uint32_t>(),
                               smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
@@ -676,6 +699,35 @@ hipError_t Context::compact(size_t i, uint32_t lo, uint32_t hi, uint32_t fill, c
     return hipStreamSynchronize(stream_);
 }
 
+hipError_t Context::partitions(size_t inst, const uint2 **rp, const uint2 **sp, std::vector<uint32_t> &pstart_r,
+                               std::vector<uint32_t> &pstart_s)
+{
+    if (inst >= h_insts_.size() || !partition_only_) return hipErrorInvalidValue;
+    const InstDesc &d = h_insts_[inst];
+    const size_t F = size_t(1) << d.bits;
+    pstart_r.resize(F + 1);
+    pstart_s.resize(F + 1);
+    SPJ_TRY(hipMemcpyAsync(pstart_r.data(), pstart_.as<uint32_t>() + d.pstartR, (F + 1) * 4, hipMemcpyDeviceToHost,
+                           stream_));
+    SPJ_TRY(hipMemcpyAsync(pstart_s.data(), pstart_.as<uint32_t>() + d.pstartS, (F + 1) * 4, hipMemcpyDeviceToHost,
+                           stream_));
+    SPJ_TRY(hipStreamSynchronize(stream_));
+    *rp = d.Rp;
+    *sp = d.Sp;
+    return hipSuccess;
+}
+
+hipError_t Context::gather(uint2 *out, const std::vector<uint4> &segs, uint64_t total)
+{
+    if (segs.empty()) return hipSuccess;
+    SPJ_TRY(tmp_.ensure(total * 8 + 8));
+    SPJ_TRY(segbuf_.ensure(segs.size() * sizeof(uint4) + 16));
+    SPJ_TRY(hipMemcpyAsync(segbuf_.p, segs.data(), segs.size() * sizeof(uint4), hipMemcpyHostToDevice, stream_));
+    SPJ_TRY(launch_gather_segments(out, tmp_.as<uint2>(), segbuf_.as<uint4>(), (uint32_t)segs.size(), stream_));
+    SPJ_TRY(hipMemcpyAsync(out, tmp_.p, total * 8, hipMemcpyDeviceToDevice, stream_));
+    return hipStreamSynchronize(stream_);
+}
+
 hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector<uint32_t> &errs, PhaseTimes *timed)
 {
     h_ctrl_.resize(ctrl_words(n));
@@ -703,7 +755,9 @@ hipError_t Context::collect(size_t n, std::vector<uint32_t> &counts, std::vector
         const uint32_t fill = h_ctrl_[ctrl_at(n, kCtrlFill, i)];
         counts[i] = fill + hi;
         const bool dense = (fill == lo) && (hi == 0 || (uint64_t)lo + hi == h_insts_[i].nS);
-        if (dense || errs[i]) continue;
+        // a first pass of two leaves its heavy pairs at the back: the
+        // second pass writes the front
+        if (dense || errs[i] || partition_only_) continue;
         if (fill != lo && !holes_read) {
             const uint32_t nh = h_ctrl_[2];
             if (nh > max_holes_) {

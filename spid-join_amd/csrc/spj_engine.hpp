@@ -24,6 +24,7 @@ struct Resolved {
     uint32_t smax = 131072;   // S tuples per work item
     uint32_t cells = 2;       // S layout: 0 counted (histogram pass), 1 cells (none), 2 auto
     uint32_t cell_pct = 0;    // cell capacity in % of the mean cell fill (0 = automatic)
+    uint32_t passes = 1;      // radix partition passes: 1, or 2 (bits is the first pass's)
 };
 
 struct InstSpec {
@@ -40,6 +41,12 @@ struct InstSpec {
     const uint32_t *pst_pre = nullptr;
     // R must end up in contiguous partitions (export_partitioned_r)
     bool r_contiguous = false;
+    // digit = (SuperFastHash(key) >> shift) & (2^cfg.bits - 1): a second
+    // radix pass over one first-pass partition takes the next bits
+    uint32_t shift = 0;
+    // partition R and S only (counted layout, heavy pairs emitted to the
+    // back of out), no join: the first of two passes.  A batch is all or none.
+    bool partition_only = false;
 };
 
 // Milliseconds per phase (HIP events on the context's stream).
@@ -49,6 +56,7 @@ struct PhaseTimes {
     uint32_t heavy_keys = 0;            // instance 0
     uint64_t r_parted = 0, s_parted = 0;  // instance 0: tuples written to partitions
     uint32_t s_cells = 0, r_cells = 0;    // instance 0: S / R layout was cells
+    uint32_t passes = 1;                  // radix partition passes
 };
 
 struct DevBuf {
@@ -114,6 +122,17 @@ public:
     // its F + 1 partition starts to caller-owned device buffers.
     hipError_t export_partitioned_r(size_t inst, uint2 *rp_dst, uint32_t *pstart_dst);
 
+    // After run()/collect() of a partition_only batch: instance i's
+    // partitioned R and S (in this context's buffers, valid until its next
+    // run) and their F + 1 partition starts, read back to the host.
+    hipError_t partitions(size_t inst, const uint2 **rp, const uint2 **sp, std::vector<uint32_t> &pstart_r,
+                          std::vector<uint32_t> &pstart_s);
+
+    // Make out[0, total) dense: copy segments (src, dst, len) of out, in
+    // order, through a temporary buffer (sources and destinations may
+    // overlap).  On this context's stream, synchronised.
+    hipError_t gather(uint2 *out, const std::vector<uint4> &segs, uint64_t total);
+
 private:
     hipError_t enqueue_plan(const std::vector<InstSpec> &specs, PhaseTimes *timed);
     void drop_graph();
@@ -127,6 +146,7 @@ private:
     hipEvent_t ev_[8] = {};   // [0, 6]: phase boundaries; [7]: end of R's own scatter launch
     bool split_timed_ = false;  // the last timed enqueue recorded ev_[7]
     bool hist_timed_ = false;   // ... ev_[2] (it launched a histogram pass)
+    bool partition_only_ = false;  // the last enqueue partitioned only (no join)
     bool scan_timed_ = false;   // ... ev_[3] (it launched a scan)
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy

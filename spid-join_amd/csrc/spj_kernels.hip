@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t sfh_low(uint32_t key)
     h += h >> 6;
     return h;
 }
-static_assert(kMaxBits <= 19, "sfh_low gives digits of up to 19 bits");
+static_assert(kMaxBits <= 19 && kMaxDigitBits <= 19, "sfh_low gives digits of up to 19 bits");
 
 // Bucket of a key in the join's LDS table: Fibonacci (multiply-shift)
 // hashing of the key.  Partitions are chosen by SuperFastHash bits, so the
@@ -118,8 +118,9 @@ __device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
     return ((key ^ 0x5BD1E995u) * 0x85EBCA77u) >> (32 - kHeavyBucketBits);
 }
 
-// Sub-table of a partition that needs Q > 1 LDS tables: high SFH bits, which
-// do not depend on the radix bits that chose the partition.
+// Sub-table of a partition that needs Q > 1 LDS tables: SFH bits above the
+// digit bits of every pass (bits = shift + radix bits), so they do not
+// depend on the bits that chose the partition.
 __device__ __forceinline__ uint32_t sub_of(uint32_t h, uint32_t bits, uint32_t Q)
 {
     return __umulhi((h >> bits) * 0x9E3779B1u, Q);
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
     const uint32_t t = xcd_tile(blockIdx.x, ntiles);
     const TileDesc td = tiles[t];
     const InstDesc &I = insts[td.inst];
-    const uint32_t F = 1u << I.bits, mask = F - 1u;
+    const uint32_t F = 1u << I.bits, mask = F - 1u, shift = I.shift;
     uint32_t *hk = lds;                   // kHeavySlots (16-byte aligned)
     uint32_t *hist = hk + kHeavySlots;    // F
     const bool heavy = I.heavy != 0 && heavy_keys_of(I) != 0;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
             if (mode == 2 && hs[j] >= 0) continue;   // heavy S tuple: no partition
-            atomicAdd(&hist[sfh_low(v[j].x) & mask], 1u);
+            atomicAdd(&hist[(sfh_low(v[j].x) >> shift) & mask], 1u);
             if (mode == 1 && hs[j] >= 0) heavy_record_rid(I, (uint32_t)hs[j], v[j].y);
         }
     }
@@ -463,15 +464,15 @@ __device__ __forceinline__ void load_tile_round(uint2 *v, const InstDesc *__rest
 // outp[bs[digit] + i] (the digit is recomputed from the key, which is
 // cheaper than staging it), heavy pairs stage[hstart, total) -- whose key
 // was replaced by the R rid -- go to out[hb + i - hstart].
-__device__ __forceinline__ void scatter_write_out(const uint2 *stage, const uint32_t *bs, uint32_t mask,
-                                                  uint32_t hstart, uint32_t total, uint2 *outp, uint2 *out,
+__device__ __forceinline__ void scatter_write_out(const uint2 *stage, const uint32_t *bs, uint32_t shift,
+                                                  uint32_t mask, uint32_t hstart, uint32_t total, uint2 *outp, uint2 *out,
                                                   uint32_t hb, uint32_t nS, uint32_t *err_inst)
 {
     uint32_t i0 = threadIdx.x;   // opaque: the stage addresses are formed here, not held across the round loop
     asm volatile("" : "+v"(i0));
     for (uint32_t i = i0; i < hstart; i += kScatterNT) {
         const uint2 t = stage[i];
-        gst(outp, bs[sfh_low(t.x) & mask] + i, t);
+        gst(outp, bs[(sfh_low(t.x) >> shift) & mask] + i, t);
     }
     for (uint32_t i = hstart + i0; i < total; i += kScatterNT) {
         const uint32_t oi = hb + (i - hstart);
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         for (int j = 0; j < IPT; j++) v[j] = nv[j];
 
         const InstDesc &I = insts[td.inst];
-        const uint32_t F = 1u << I.bits, mask = F - 1u;
+        const uint32_t F = 1u << I.bits, mask = F - 1u, shift = I.shift;
         const bool heavy = (I.heavy != 0) && (td.rel == 1) && heavy_keys_of(I) != 0;
         // R tiles of the cell layout record each heavy key's R rid (in the
         // counted layout the histogram pass does); S tiles read the rids in
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
                 const uint32_t li = j * NT + threadIdx.x;
                 dg[j] = DROP;
                 if (li < rn && v[j].x != kEmptyKey) {
-                    if (hs[j] < 0) dg[j] = sfh_low(v[j].x) & mask;
+                    if (hs[j] < 0) dg[j] = (sfh_low(v[j].x) >> shift) & mask;
                     else { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
                 }
                 htot += (uint32_t)__popcll(__ballot(dg[j] == F));
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
-                dg[j] = (li < rn && v[j].x != kEmptyKey) ? (sfh_low(v[j].x) & mask) : DROP;
+                dg[j] = (li < rn && v[j].x != kEmptyKey) ? ((sfh_low(v[j].x) >> shift) & mask) : DROP;
             }
         }
 #pragma unroll
@@ -721,7 +722,7 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
             misc[0] = hn ? I.nS - hres - hn : 0u;
         }
         __syncthreads();
-        scatter_write_out(stage, bs, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
+        scatter_write_out(stage, bs, shift, mask, hstart, heavy ? total : hstart, td.rel ? I.Sp : I.Rp, I.out, misc[0], I.nS,
                           &err[td.inst]);
         if (!more) break;
         par = !par;
@@ -1666,7 +1667,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                 }
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
-                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.shift + I.bits, Q) != q) continue;
                                     atomicAdd(&bcnt[bucket_of(c[u].x, lb)], 1u);
                                 }
                             }
@@ -1769,7 +1770,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                 }
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
-                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.bits, Q) != q) continue;
+                                    if (c[u].x == kEmptyKey || sub_of(sfh(c[u].x), I.shift + I.bits, Q) != q) continue;
                                     const uint32_t b = bucket_of(c[u].x, lb), sh16 = (b & 1u) * 16u;
                                     const uint32_t pos = (atomicAdd(&ends_w[b >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
                                     tkv[pos] = c[u];

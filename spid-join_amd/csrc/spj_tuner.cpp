@@ -117,6 +117,7 @@ int spj_tuner_get(uint64_t nR, uint64_t nS, double zipf, spj_config_t *cfg)
     cfg->items_s_max = r.smax;
     cfg->part_layout = r.cells == 2 ? -1 : (int32_t)r.cells;
     cfg->cell_capacity_pct = r.cell_pct ? (int32_t)r.cell_pct : -1;
+    cfg->radix_passes = (int32_t)r.passes;
     return SPJ_OK;
 }
 
@@ -138,11 +139,11 @@ int spj_tuner_save(const char *path)
     FILE *f = fopen(path, "w");
     if (!f) return fail(SPJ_EINVAL, "cannot write %s", path);
     std::lock_guard<std::mutex> lk(g_mu);
-    fprintf(f, "# log2R ratio_log2 skew bits table_log2 alg ipt tile smax heavy_min\n");
+    fprintf(f, "# log2R ratio_log2 skew bits table_log2 alg ipt tile smax heavy_min passes\n");
     for (auto &kv : g_table) {
         const Resolved &r = kv.second;
-        fprintf(f, "%u %u %u %u %u %u %u %u %u %u\n", std::get<0>(kv.first), std::get<1>(kv.first),
-                std::get<2>(kv.first), r.bits, r.table_log2, r.alg, r.ipt, r.tile, r.smax, r.heavy_min);
+        fprintf(f, "%u %u %u %u %u %u %u %u %u %u %u\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                std::get<2>(kv.first), r.bits, r.table_log2, r.alg, r.ipt, r.tile, r.smax, r.heavy_min, r.passes);
     }
     fclose(f);
     return SPJ_OK;
@@ -156,10 +157,12 @@ int spj_tuner_load(const char *path)
     std::map<Key, Resolved> t;
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
-        unsigned a, b, c, bits, tl, alg, ipt, tile, smax, hm;
-        if (sscanf(line, "%u %u %u %u %u %u %u %u %u %u", &a, &b, &c, &bits, &tl, &alg, &ipt, &tile, &smax, &hm) != 10)
-            continue;
+        unsigned a, b, c, bits, tl, alg, ipt, tile, smax, hm, passes = 1;   // (tables saved without passes: one)
+        const int got = sscanf(line, "%u %u %u %u %u %u %u %u %u %u %u", &a, &b, &c, &bits, &tl, &alg, &ipt, &tile,
+                               &smax, &hm, &passes);
+        if (got != 10 && got != 11) continue;
         Resolved r;
+        r.passes = passes;
         r.bits = bits;
         r.table_log2 = tl;
         r.alg = alg;
@@ -188,6 +191,8 @@ int spj_tuner_load(const char *path)
  *   scatter ipt    8, 10, 12 tuples per thread per scatter round
  *   table buckets  2^12, 2^13
  *   S per item     2^16, 2^17, 2^18 S tuples per join work item
+ *   passes         1 or 2 radix passes, where one pass of the most bits
+ *                  leaves R partitions larger than an LDS table
  * and the sweep runs twice.  Each candidate runs reps + 1 times (the first
  * is a warm-up); its score is the median device time (events around the
  * launch sequence).  Workgroup sizes are not searched: the scatter and join
@@ -204,7 +209,7 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
     std::lock_guard<std::mutex> lk(default_context_mutex());
     std::map<std::vector<uint32_t>, float> seen;   // candidate -> median ms
     auto score = [&](const Resolved &c, float *ms) -> int {
-        const std::vector<uint32_t> key = {c.alg, c.bits, c.ipt, c.table_log2, c.smax};
+        const std::vector<uint32_t> key = {c.alg, c.bits, c.ipt, c.table_log2, c.smax, c.passes};
         auto it = seen.find(key);
         if (it != seen.end()) {
             *ms = it->second;
@@ -218,12 +223,12 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
         s.nS = (uint32_t)nS;
         s.cfg = c;
         std::vector<float> ts;
-        std::vector<uint32_t> counts, errs;
         for (int i = 0; i < reps + 1; i++) {
             PhaseTimes t;
-            hipError_t e = default_context().run({s}, counts, errs, &t);
+            uint32_t count = 0, err = 0;
+            hipError_t e = run_one(s, count, err, &t);
             if (e != hipSuccess) return hip_fail(e, "autotune run");
-            if (errs[0]) break;   // e.g. a duplicate key: no score
+            if (err) break;   // e.g. a duplicate key: no score
             if (i) ts.push_back(t.total);
         }
         float med = 1e30f;
@@ -243,7 +248,8 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
         for (uint32_t v : vals) {
             Resolved c = win;
             set(c, v);
-            if (c.alg == SPJ_ALG_PHJ && (nR >> c.bits) > 4ull * join_table_cap()) continue;
+            if (c.alg == SPJ_ALG_PHJ && c.passes == 1 && (nR >> c.bits) > 4ull * join_table_cap()) continue;
+            if (c.passes == 2 && (c.alg == SPJ_ALG_NPHJ || c.bits == 0)) continue;
             if (c.alg == SPJ_ALG_NPHJ && nR > (uint64_t(1) << 24)) continue;
             float ms = 0;
             int r = score(c, &ms);
@@ -268,6 +274,8 @@ int spj_autotune(const spj_tuple_t *R, uint64_t nR, const spj_tuple_t *S, uint64
         if ((rc = try_dim([](Resolved &c, uint32_t v) { c.ipt = v; }, {8u, 10u, 12u}))) return rc;
         if ((rc = try_dim([](Resolved &c, uint32_t v) { c.table_log2 = v; }, {12u, 13u}))) return rc;
         if ((rc = try_dim([](Resolved &c, uint32_t v) { c.smax = v; }, {1u << 16, 1u << 17, 1u << 18}))) return rc;
+        if ((nR >> kMaxBits) > join_table_cap())
+            if ((rc = try_dim([](Resolved &c, uint32_t v) { c.passes = v; }, {1u, 2u}))) return rc;
     }
     {
         std::lock_guard<std::mutex> lk2(g_mu);

@@ -44,14 +44,15 @@ class Config(ctypes.Structure):
     _fields_ = [("algorithm", ctypes.c_int32), ("radix_bits", ctypes.c_int32), ("table_log2", ctypes.c_int32),
                 ("heavy", ctypes.c_int32), ("heavy_min_count", ctypes.c_int32), ("scatter_ipt", ctypes.c_int32),
                 ("tile_tuples", ctypes.c_int64), ("items_s_max", ctypes.c_int64),
-                ("part_layout", ctypes.c_int32), ("cell_capacity_pct", ctypes.c_int32)]
+                ("part_layout", ctypes.c_int32), ("cell_capacity_pct", ctypes.c_int32),
+                ("radix_passes", ctypes.c_int32)]
 
 
 class PhaseTimes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in
                 ("heavy_ms", "hist_ms", "scan_ms", "scatter_ms", "items_ms", "join_ms", "total_ms")] + [
         ("heavy_keys", ctypes.c_uint32), ("r_partitioned", ctypes.c_uint64), ("s_partitioned", ctypes.c_uint64),
-        ("s_cells", ctypes.c_uint32), ("r_cells", ctypes.c_uint32)]
+        ("s_cells", ctypes.c_uint32), ("r_cells", ctypes.c_uint32), ("passes", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))
