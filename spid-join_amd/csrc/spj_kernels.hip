@@ -1808,42 +1808,36 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             for (uint32_t bb = b0; bb < nb; bb += NT) {
                                 const uint32_t st = bend[bb], n = bend[bb + 1] - st;
                                 if (n <= 1) continue;
-                                if (n <= 4) {
-                                    // (key, position) as 64-bit words, so that the
-                                    // order is strict; slots past n sort last
-                                    uint64_t e[4];
-                                    uint32_t rid[4];
+                                if (n <= 8) {
+                                    // (key, rid) as 64-bit words, slots past n sort
+                                    // last; Batcher's odd-even merge network of 19
+                                    // comparators.  A bucket up to 8 entries long
+                                    // (1 in 4000 are longer at 2 entries per bucket
+                                    // on average) never waits on dependent LDS reads.
+                                    // Slots past the bucket may lie past the table's
+                                    // slack, in the bucket ends: read, never used.
+                                    uint64_t e[8];
 #pragma unroll
-                                    for (uint32_t u = 0; u < 4; u++) {
-                                        const uint2 t = tkv[st + u];   // the table has kScan slack slots
-                                        e[u] = u < n ? ((uint64_t)t.x << 3) | u : ~0ull;
-                                        rid[u] = t.y;
+                                    for (uint32_t u = 0; u < 8; u++) {
+                                        const uint2 t = tkv[st + u];
+                                        e[u] = u < n ? ((uint64_t)t.x << 32) | t.y : ~0ull;
                                     }
                                     auto cx = [&](int a, int c) {
                                         const uint64_t x = e[a], y = e[c];
                                         e[a] = x < y ? x : y;
                                         e[c] = x < y ? y : x;
                                     };
-                                    cx(0, 1);
-                                    cx(2, 3);
-                                    cx(0, 2);
-                                    cx(1, 3);
-                                    cx(1, 2);
-                                    uint2 o[4];
+                                    cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+                                    cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+                                    cx(1, 2); cx(5, 6);
+                                    cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+                                    cx(2, 4); cx(3, 5);
+                                    cx(1, 2); cx(3, 4); cx(5, 6);
 #pragma unroll
-                                    for (uint32_t u = 0; u < 4; u++) {
-                                        const uint32_t src = (uint32_t)e[u] & 7u;
-                                        uint32_t rr = rid[0];
-                                        rr = src == 1u ? rid[1] : rr;
-                                        rr = src == 2u ? rid[2] : rr;
-                                        rr = src == 3u ? rid[3] : rr;
-                                        o[u] = make_uint2((uint32_t)(e[u] >> 3), rr);
-                                    }
+                                    for (uint32_t u = 0; u < 8; u++)
+                                        if (u < n) tkv[st + u] = make_uint2((uint32_t)(e[u] >> 32), (uint32_t)e[u]);
 #pragma unroll
-                                    for (uint32_t u = 0; u < 4; u++)
-                                        if (u < n) tkv[st + u] = o[u];
-#pragma unroll
-                                    for (uint32_t u = 1; u < 4; u++) dup |= u < n && o[u].x == o[u - 1].x;
+                                    for (uint32_t u = 1; u < 8; u++) dup |= u < n && (e[u] >> 32) == (e[u - 1] >> 32);
                                 } else if (n <= kRangeSortMax) {
                                     for (uint32_t a = st + 1; a < st + n; a++) {   // insertion sort (rare)
                                         const uint2 t = tkv[a];
