@@ -112,11 +112,15 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_bucket
 // selections without a slot (it stays an ordinary key).
 constexpr uint32_t kHeavyBucketBits = 9;    // 512 buckets x 4 slots = kHeavySlots
 static_assert(kHeavySlots == (4u << kHeavyBucketBits), "heavy table: 4-slot buckets");
-__device__ __forceinline__ uint32_t heavy_b1(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - kHeavyBucketBits); }
-__device__ __forceinline__ uint32_t heavy_b2(uint32_t key)
-{
-    return ((key ^ 0x5BD1E995u) * 0x85EBCA77u) >> (32 - kHeavyBucketBits);
-}
+// The two buckets of a key come from its SuperFastHash h = sfh(key), which
+// the scatter computes for the digit anyway: bits 13..21 and 23..31, above
+// the digit bits of any pass that uses the heavy table (radix bits <= 12).
+// Two bit fields cost two VALU operations where multiplicative hashes cost
+// two quarter-rate 32-bit multiplies and more.
+static_assert(kMaxBits <= 13 && kHeavyBucketBits <= 9, "heavy buckets: sfh bits 13..21 and 23..31");
+constexpr int kHeavyLookupGroup = 4;   // scatter: tuples whose heavy lookups are issued together
+__device__ __forceinline__ uint32_t heavy_b1(uint32_t h) { return (h >> 13) & ((1u << kHeavyBucketBits) - 1u); }
+__device__ __forceinline__ uint32_t heavy_b2(uint32_t h) { return h >> (32 - kHeavyBucketBits); }
 
 // Sub-table of a partition that needs Q > 1 LDS tables: SFH bits above the
 // digit bits of every pass (bits = shift + radix bits), so they do not
@@ -221,31 +225,29 @@ __device__ __forceinline__ void heavy_record_rid(const InstDesc &I, uint32_t s, 
     atomicOr(I.heavy_rids + kHeavySlots + (s >> 5), 1u << (s & 31u));
 }
 
-// S side: whether slot s (>= 0) has an R tuple, from the presence bitmap
-// in LDS behind the rids.
-__device__ __forceinline__ bool heavy_present(const uint32_t *hr, int s)
-{
-    return (hr[kHeavySlots + ((uint32_t)s >> 5)] >> ((uint32_t)s & 31u)) & 1u;
-}
-
-// Slot of key (never 0: empty slots hold key 0) in the LDS heavy table, or
-// -1.  Both candidate buckets are read at once and the slot is selected
-// without branches.  hk is 16-byte aligned.
-__device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key)
+// Slot of key (never 0: empty slots hold key 0; h = sfh(key)) in the LDS
+// heavy table, or -1 when it is not there or valid is false.  Both
+// candidate buckets are read at once -- also for an invalid tuple (past the
+// round's end, or the empty key, which would match an empty slot): the
+// buckets are in range for any h, so a thread's lookups need no branch and
+// their reads go out together.  The matching position among the 8 slots is
+// chosen with constants, and the slot formed once.  hk is 16-byte aligned.
+__device__ __forceinline__ int heavy_find(const uint32_t *hk, uint32_t key, uint32_t h, bool valid)
 {
     const uint4 *hb = reinterpret_cast<const uint4 *>(hk);
-    const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
+    const uint32_t b1 = heavy_b1(h), b2 = heavy_b2(h);
     const uint4 a = hb[b1], c = hb[b2];
-    int s = -1;
-    s = (c.w == key) ? (int)(4 * b2 + 3) : s;
-    s = (c.z == key) ? (int)(4 * b2 + 2) : s;
-    s = (c.y == key) ? (int)(4 * b2 + 1) : s;
-    s = (c.x == key) ? (int)(4 * b2) : s;
-    s = (a.w == key) ? (int)(4 * b1 + 3) : s;
-    s = (a.z == key) ? (int)(4 * b1 + 2) : s;
-    s = (a.y == key) ? (int)(4 * b1 + 1) : s;
-    s = (a.x == key) ? (int)(4 * b1) : s;
-    return s;
+    uint32_t i = 8;
+    i = (c.w == key) ? 7u : i;
+    i = (c.z == key) ? 6u : i;
+    i = (c.y == key) ? 5u : i;
+    i = (c.x == key) ? 4u : i;
+    i = (a.w == key) ? 3u : i;
+    i = (a.z == key) ? 2u : i;
+    i = (a.y == key) ? 1u : i;
+    i = (a.x == key) ? 0u : i;
+    const int s = (int)(4u * (i < 4u ? b1 : b2) + (i & 3u));
+    return (valid && i < 8u) ? s : -1;
 }
 
 // Tile t of the list from its run (runs sorted by first, none empty).
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
         // heavy lookups of all U tuples first, so their LDS reads overlap
         int hs[U];
 #pragma unroll
-        for (uint32_t j = 0; j < U; j++) hs[j] = (mode != 0 && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
+        for (uint32_t j = 0; j < U; j++) hs[j] = mode != 0 ? heavy_find(hk, v[j].x, sfh(v[j].x), v[j].x != kEmptyKey) : -1;
 #pragma unroll
         for (uint32_t j = 0; j < U; j++) {
             if (v[j].x == kEmptyKey) continue;
@@ -575,32 +577,45 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
         uint32_t dg[IPT], rk[IPT];
         if (heavy) {
             // in phases, so that the LDS reads of all IPT tuples overlap:
-            // table lookups, then R rids, then one rank atomic per wave for
-            // the heavy bucket F, then the digit atomics below
+            // table lookups (keyed by the SuperFastHash that also gives the
+            // digit), then each slot's rid and presence word -- read for
+            // every tuple, slot 0 for the others, so that no read waits
+            // behind a branch -- then one rank atomic per wave for the heavy
+            // bucket F, then the digit atomics below
             int hs[IPT];
+            uint32_t hh[IPT];
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
-                hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
+                hh[j] = sfh(v[j].x);
+                hs[j] = heavy_find(hk, v[j].x, hh[j], li < rn && v[j].x != kEmptyKey);
+                // lookups in groups: the empty asm (memory clobber) keeps
+                // the next group's reads below this point, which bounds the
+                // registers in flight (all 12 at once spill)
+                if ((j + 1) % kHeavyLookupGroup == 0)
+                    for (int u = j + 1 - kHeavyLookupGroup; u <= j; u++) asm volatile("" : "+v"(hs[u])::"memory");
             }
-            // a heavy key with no R tuple: the tuple is dropped, as one
-            // with the empty key 0 (it was not counted either)
-#pragma unroll
-            for (int j = 0; j < IPT; j++)
-                if (hs[j] >= 0 && !heavy_present(hr, hs[j])) v[j].x = kEmptyKey;
-            uint32_t hrid[IPT];
-#pragma unroll
-            for (int j = 0; j < IPT; j++) hrid[j] = (hs[j] >= 0) ? hr[hs[j]] : 0u;
             uint32_t htot = 0;
 #pragma unroll
             for (int j = 0; j < IPT; j++) {
                 const uint32_t li = j * NT + threadIdx.x;
-                dg[j] = DROP;
-                if (li < rn && v[j].x != kEmptyKey) {
-                    if (hs[j] < 0) dg[j] = (sfh_low(v[j].x) >> shift) & mask;
-                    else { dg[j] = F; v[j].x = hrid[j]; }   // heavy: bucket F
-                }
-                htot += (uint32_t)__popcll(__ballot(dg[j] == F));
+                const uint32_t sl = hs[j] >= 0 ? (uint32_t)hs[j] : 0u;
+                const uint32_t pw = hr[kHeavySlots + (sl >> 5)], rid = hr[sl];
+                // a heavy key with no R tuple: the tuple is dropped, as one
+                // with the empty key 0 (it was not counted either); selects
+                // only, so that no read is moved into a branch of its own
+                const uint32_t present = __builtin_amdgcn_ubfe(pw, sl, 1u);   // bit sl & 31
+                const uint32_t dig = (hh[j] >> shift) & mask;                // sfh's bits 0..18 are sfh_low's
+                // (the selects on loaded values are written as arithmetic:
+                // as ?: the compiler makes them branches, and moves the
+                // presence read into the branch with a wait of its own)
+                const uint32_t hd = DROP ^ ((0u - present) & (F ^ DROP));                    // present ? F : DROP
+                const uint32_t found = (uint32_t)hs[j] >> 31 ^ 1u;                          // hs[j] >= 0
+                const uint32_t inner = dig ^ ((0u - found) & (hd ^ dig));                   // found ? hd : dig
+                const uint32_t d = inner | (0u - (uint32_t)!((li < rn) & (v[j].x != kEmptyKey)));   // DROP when not valid
+                dg[j] = d;
+                v[j].x = d == F ? rid : v[j].x;   // heavy: bucket F, staged as (R rid, S rid)
+                htot += (uint32_t)__popcll(__ballot(d == F));
             }
             if (htot) {
                 uint32_t hb = 0;
@@ -621,7 +636,9 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
 #pragma unroll
                 for (int j = 0; j < IPT; j++) {
                     const uint32_t li = j * NT + threadIdx.x;
-                    hs[j] = (li < rn && v[j].x != kEmptyKey) ? heavy_find(hk, v[j].x) : -1;
+                    hs[j] = heavy_find(hk, v[j].x, sfh(v[j].x), li < rn && v[j].x != kEmptyKey);
+                    if ((j + 1) % kHeavyLookupGroup == 0)
+                        for (int u = j + 1 - kHeavyLookupGroup; u <= j; u++) asm volatile("" : "+v"(hs[u])::"memory");
                 }
 #pragma unroll
                 for (int j = 0; j < IPT; j++)
@@ -991,7 +1008,7 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     for (uint32_t r0 = 0; r0 < nkeys; r0 += kWaveKeys) {
         if (threadIdx.x < kWaveKeys && r0 + threadIdx.x < nkeys) {
             const uint32_t key = (uint32_t)ord[m - 1 - (r0 + threadIdx.x)];
-            const uint32_t b1 = heavy_b1(key), b2 = heavy_b2(key);
+            const uint32_t h = sfh(key), b1 = heavy_b1(h), b2 = heavy_b2(h);
             const uint32_t first = bfill[b2] < bfill[b1] ? b2 : b1, second = first == b1 ? b2 : b1;
             uint32_t slot = atomicAdd(&bfill[first], 1u), b = first;
             if (slot >= 4 && second != first) {
