@@ -453,12 +453,15 @@ __device__ __forceinline__ void load_tile_round(uint2 *v, const InstDesc *__rest
 {
     constexpr uint32_t NT = kScatterNT;
     const InstDesc &I = insts[td.inst];
-    const uint2 *in = td.rel ? I.S : I.R;
+    // the round's base is uniform: with it in scalar registers each load
+    // takes the 32-bit lane offset (global_load v, s[base]) instead of a
+    // 64-bit address formed per load
+    const uint2 *in = (td.rel ? I.S : I.R) + o;
     const uint32_t rn = min((uint32_t)(NT * IPT), td.end - o);
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         const uint32_t li = j * NT + threadIdx.x;
-        if (li < rn) v[j] = gld(in, o + li);
+        if (li < rn) v[j] = gld(in, li);
     }
 }
 
@@ -769,6 +772,7 @@ constexpr uint32_t kGateMin = 8;          // pre-sample count of one key that op
 constexpr uint32_t kSampleSlots = 16384;
 constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
+static_assert(kCandSlots % 1024 == 0, "k_heavy_select reads the candidate map 1024 slots at a time");
 
 __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                       uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase,
@@ -832,7 +836,12 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
             const uint32_t key = sk[i];
             uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
             for (uint32_t step = 0; step < kCandSlots; step++) {
-                uint32_t old = atomicCAS(&cand_keys[s], kEmptyKey, key);
+                // a slot holding a key never changes, so a plain read that
+                // sees one is exact; only a slot seen empty (possibly stale)
+                // takes the CAS.  Hot keys, merged by every sample block,
+                // then cost one atomic instead of two.
+                uint32_t old = gld32(cand_keys, s);
+                if (old == kEmptyKey) old = atomicCAS(&cand_keys[s], kEmptyKey, key);
                 if (old == kEmptyKey || old == key) {
                     const uint32_t sum = atomicAdd(&cand_cnt[s], sc[i]) + sc[i];
                     // open the second phase only on a key seen often across the
@@ -985,10 +994,22 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     for (uint32_t i = threadIdx.x; i < kHeavySlots / 4; i += blockDim.x) bfill[i] = 0;
     for (uint32_t i = threadIdx.x; i < kSel; i += blockDim.x) ord[i] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kCandSlots; i += blockDim.x) {
-        if (cand_keys[i] == kEmptyKey || cand_cnt[i] < min_count) continue;
-        const uint32_t k = atomicAdd(&n, 1u);
-        if (k < kSel) ord[k] = ((uint64_t)cand_cnt[i] << 32) | cand_keys[i];
+    {
+        // all of a thread's slots are loaded first (one wait, not one per
+        // slot), then the candidates that reach min_count are appended
+        constexpr uint32_t SPT = kCandSlots / 1024;
+        uint32_t ck[SPT], cc[SPT];
+#pragma unroll
+        for (uint32_t u = 0; u < SPT; u++) {
+            ck[u] = gld32(cand_keys, u * 1024 + threadIdx.x);
+            cc[u] = gld32(cand_cnt, u * 1024 + threadIdx.x);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SPT; u++) {
+            if (ck[u] == kEmptyKey || cc[u] < min_count) continue;
+            const uint32_t k = atomicAdd(&n, 1u);
+            if (k < kSel) ord[k] = ((uint64_t)cc[u] << 32) | ck[u];
+        }
     }
     __syncthreads();
     const uint32_t nsel = min(n, kSel);
