@@ -774,6 +774,34 @@ constexpr uint32_t kLocalMin = 4;
 constexpr uint32_t kCandSlots = 16384;
 static_assert(kCandSlots % 1024 == 0, "k_heavy_select reads the candidate map 1024 slots at a time");
 
+constexpr uint32_t kAccSlots = 2048;      // a workgroup's candidates over its sample blocks (LDS)
+
+// Add cnt sampled occurrences of key to the job's global candidate map.
+// A slot holding a key never changes, so a plain read that sees one is
+// exact; only a slot seen empty (possibly stale) takes the CAS.  Phase 0
+// needs the sum for its gate; phase 1 adds without a return value, so
+// nothing waits for it.
+__device__ __forceinline__ void cand_merge(uint32_t *cand_keys, uint32_t *cand_cnt, uint32_t *found, uint32_t key,
+                                           uint32_t cnt, uint32_t phase)
+{
+    uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
+    for (uint32_t step = 0; step < kCandSlots; step++) {
+        uint32_t old = gld32(cand_keys, s);
+        if (old == kEmptyKey) old = atomicCAS(&cand_keys[s], kEmptyKey, key);
+        if (old == kEmptyKey || old == key) {
+            if (phase == 0) {
+                // open the second phase only on a key seen often across the
+                // pre-sample, not on one chance cluster in a single block
+                if (atomicAdd(&cand_cnt[s], cnt) + cnt >= kGateMin) *found = 1u;
+            } else {
+                atomicAdd(&cand_cnt[s], cnt);
+            }
+            return;
+        }
+        s = (s + 1u) & (kCandSlots - 1u);
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restrict__ insts, const HeavyJob *__restrict__ jobs,
                                                       uint32_t njobs, uint32_t *__restrict__ cand_base, uint32_t phase,
                                                       uint32_t nvb)
@@ -781,6 +809,25 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
     extern __shared__ __align__(16) uint32_t lds[];
     uint32_t *sk = lds;
     uint32_t *sc = lds + kSampleSlots;
+    // The workgroup's candidates (keys seen kLocalMin times in a block) are
+    // summed over its sample blocks of one job in LDS and merged into the
+    // job's global map once: every block holds the same hot keys, and
+    // merging each block's would put all workgroups' atomics on the same
+    // few global words at once.
+    uint32_t *ak = sc + kSampleSlots;
+    uint32_t *ac = ak + kAccSlots;
+    for (uint32_t i = threadIdx.x; i < 2 * kAccSlots; i += blockDim.x) ak[i] = 0;
+    uint32_t acc_job = 0xFFFFFFFFu;   // job whose counts ak / ac hold (uniform)
+    bool skip_job = false;            // phase 1: the job's pre-sample opened no gate
+    // each thread flushes and clears its own accumulator slots
+    auto flush = [&](uint32_t jb) {
+        uint32_t *ck = cand_base + jobs[jb].cand_off, *cc = ck + kCandSlots, *fnd = cc + kCandSlots;
+        for (uint32_t i = threadIdx.x; i < kAccSlots; i += blockDim.x) {
+            if (ac[i]) cand_merge(ck, cc, fnd, ak[i], ac[i], phase);
+            ak[i] = 0;
+            ac[i] = 0;
+        }
+    };
     // workgroups loop over the phase's nvb sample blocks (a grid of at most
     // one workgroup per CU: when phase 1 has nothing to do, its blocks exit
     // without each being dispatched)
@@ -792,18 +839,22 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
             const uint32_t m = (lo + hi) >> 1;
             if (jobs[m].first_block[phase] <= vb) lo = m; else hi = m;
         }
+        if (lo != acc_job) {   // a new job: the previous one's counts go to its map
+            if (acc_job != 0xFFFFFFFFu) flush(acc_job);
+            acc_job = lo;
+            skip_job = phase == 1 && *(cand_base + jobs[lo].cand_off + 2 * kCandSlots) == 0u;
+        }
+        // no candidate in the pre-sample: no skew worth the rest
+        if (skip_job) continue;
         // the job's fields are read where they lie (a local copy indexed by
         // phase would live in scratch memory)
         const HeavyJob &job = jobs[lo];
         const InstDesc &I = insts[job.inst];
-        uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
-        uint32_t *found = cand_cnt + kCandSlots;
         // sample block of this workgroup: phase 0 takes blocks 0, kPreSample,
         // 2 kPreSample, ...; phase 1 the others
         const uint32_t l = vb - job.first_block[phase];
         const uint32_t k = phase == 0 ? l * kPreSample : (l / (kPreSample - 1)) * kPreSample + l % (kPreSample - 1) + 1;
         if (k >= job.nblocks) continue;
-        if (phase == 1 && *found == 0) continue;   // no candidate in the pre-sample: no skew worth the rest
         const uint2 *S = I.S;   // read once: reloaded per key, each key's load would wait for it
         const uint64_t b = (uint64_t)k * job.stride * kSampleBlock;
         const uint64_t e = min<uint64_t>(I.nS, b + kSampleBlock);
@@ -831,28 +882,28 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
             }
         }
         __syncthreads();
+        uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
+        uint32_t *found = cand_cnt + kCandSlots;
         for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
-            if (sc[i] < kLocalMin) continue;
+            const uint32_t cnt = sc[i];
+            if (cnt < kLocalMin) continue;
             const uint32_t key = sk[i];
-            uint32_t s = (key * 0x9E3779B1u) >> (32 - 14);
-            for (uint32_t step = 0; step < kCandSlots; step++) {
-                // a slot holding a key never changes, so a plain read that
-                // sees one is exact; only a slot seen empty (possibly stale)
-                // takes the CAS.  Hot keys, merged by every sample block,
-                // then cost one atomic instead of two.
-                uint32_t old = gld32(cand_keys, s);
-                if (old == kEmptyKey) old = atomicCAS(&cand_keys[s], kEmptyKey, key);
+            uint32_t s = (key * 0x9E3779B1u) >> (32 - 11);
+            bool placed = false;
+            for (uint32_t step = 0; step < kAccSlots; step++) {
+                const uint32_t old = atomicCAS(&ak[s], kEmptyKey, key);
                 if (old == kEmptyKey || old == key) {
-                    const uint32_t sum = atomicAdd(&cand_cnt[s], sc[i]) + sc[i];
-                    // open the second phase only on a key seen often across the
-                    // pre-sample, not on one chance cluster in a single block
-                    if (phase == 0 && sum >= kGateMin) *found = 1u;
+                    atomicAdd(&ac[s], cnt);
+                    placed = true;
                     break;
                 }
-                s = (s + 1u) & (kCandSlots - 1u);
+                s = (s + 1u) & (kAccSlots - 1u);
             }
+            if (!placed) cand_merge(cand_keys, cand_cnt, found, key, cnt, phase);   // accumulator full
         }
     }
+    __syncthreads();   // the last block's accumulation is done
+    if (acc_job != 0xFFFFFFFFu) flush(acc_job);
 }
 
 __device__ __forceinline__ void bitonic_sort_lds(uint64_t *a, uint32_t n /* power of two */)
@@ -2335,7 +2386,7 @@ hipError_t launch_heavy_detect(const InstDesc *insts, const HeavyJob *jobs, uint
                                uint32_t *cand_base, uint32_t *nheavy, uint32_t grid, hipStream_t st)
 {
     if (!njobs || !blocks[0]) return hipSuccess;
-    size_t l = 2 * kSampleSlots * 4;
+    size_t l = (2 * kSampleSlots + 2 * kAccSlots) * 4;
     hipError_t e = set_lds((const void *)k_heavy_sample, l);
     if (e != hipSuccess) return e;
     const uint32_t g0 = min(blocks[0], grid), g1 = min(blocks[1], grid);
