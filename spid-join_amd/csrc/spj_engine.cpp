@@ -219,20 +219,21 @@ void Context::drop_graph()
 // sizes and parameters) replays the launch sequence as a HIP graph: the
 // planning, the descriptor contents and every buffer address are the same,
 // so only the host's launch work is saved.  The graph is captured at the
-// first repeat and dropped by any other call (which may move buffers).
-// Timed calls (phase events) and the counted redo always launch directly.
+// first repeat and dropped by any other call (which may move buffers).  The
+// graph holds no descriptor upload: the previous call, identical and
+// launched directly, left the same descriptors on the device, and no
+// kernel writes them.  Timed calls (phase events) and the counted redo
+// launch directly: event records captured in a graph kept the times of
+// its first replay on MI355X (ROCm 7), every later replay reported them
+// again.
 hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *timed)
 {
     if (graphs_ && !timed && !force_counted_ && last_valid_ && &specs_in != &h_specs_ && same_specs(specs_in, h_specs_)) {
         if (!graph_exec_) {
-            // the staging of the descriptors must be free before the capture
-            // rewrites it (the graph's copy node reads it at every replay)
-            if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));
-            desc_pending_ = false;
             hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed);
             if (e == hipSuccess) {
                 capturing_ = true;
-                e = enqueue_plan(h_specs_, nullptr);
+                e = enqueue_plan(h_specs_, timed);
                 capturing_ = false;
                 hipGraph_t g = nullptr;
                 const hipError_t e2 = hipStreamEndCapture(stream_, &g);
@@ -251,8 +252,6 @@ hipError_t Context::enqueue(const std::vector<InstSpec> &specs_in, PhaseTimes *t
         }
         SPJ_TRY(hipGraphLaunch(graph_exec_, stream_));
         graph_replays_++;
-        SPJ_TRY(hipEventRecord(desc_ev_, stream_));   // the replay's copy has read the staging
-        desc_pending_ = true;
         return hipSuccess;
     }
     drop_graph();
@@ -544,22 +543,24 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
 
     hipStream_t st = stream_;
     // one upload of all descriptors through page-locked staging; the
-    // staging is reused once the previous upload has left it
-    if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));   // (never pending while capturing)
-    SPJ_TRY(pin_desc_.ensure(desc_bytes));
-    {
-        char *h = pin_desc_.as<char>();
-        if (n) memcpy(h + o_insts, h_insts_.data(), n * sizeof(InstDesc));
-        if (!h_runs_.empty()) memcpy(h + o_runs, h_runs_.data(), h_runs_.size() * sizeof(TileRun));
-        if (!h_segs_.empty()) memcpy(h + o_segs, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg));
-        if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
-        if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
-        if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
-        if (!h_cells_.empty()) memcpy(h + o_cells, h_cells_.data(), h_cells_.size() * sizeof(CellGroup));
-        if (!h_nphj_.empty()) memcpy(h + o_nphj, h_nphj_.data(), h_nphj_.size() * sizeof(NphjJob));
-    }
-    SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
-    if (!capturing_) {   // a replay records it after its launch
+    // staging is reused once the previous upload has left it.  A capture
+    // uploads nothing: the identical previous call left the descriptors on
+    // the device (see enqueue).
+    if (!capturing_) {
+        if (desc_pending_) SPJ_TRY(hipEventSynchronize(desc_ev_));
+        SPJ_TRY(pin_desc_.ensure(desc_bytes));
+        {
+            char *h = pin_desc_.as<char>();
+            if (n) memcpy(h + o_insts, h_insts_.data(), n * sizeof(InstDesc));
+            if (!h_runs_.empty()) memcpy(h + o_runs, h_runs_.data(), h_runs_.size() * sizeof(TileRun));
+            if (!h_segs_.empty()) memcpy(h + o_segs, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg));
+            if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
+            if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
+            if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
+            if (!h_cells_.empty()) memcpy(h + o_cells, h_cells_.data(), h_cells_.size() * sizeof(CellGroup));
+            if (!h_nphj_.empty()) memcpy(h + o_nphj, h_nphj_.data(), h_nphj_.size() * sizeof(NphjJob));
+        }
+        SPJ_TRY(hipMemcpyAsync(desc_.p, pin_desc_.p, desc_bytes, hipMemcpyHostToDevice, st));
         SPJ_TRY(hipEventRecord(desc_ev_, st));
         desc_pending_ = true;
     }

@@ -40,6 +40,19 @@ def test_graph_replays_match_eager_launches(gpu, orc, nR, nS, zipf, alg):
         run_and_check(sj, dR, nR, dS, nS, dO, cfg, want, 6, scratch)
         c1, r1 = sj.graph_stats()
         assert c1 == c0 + 1 and r1 - r0 == 6   # captured at the first repeat of the eager call, replayed by all
+        # a timed repeat launches directly (its phase events are recorded
+        # anew), and the untimed call after it captures again
+        ts = []
+        for k in range(2):
+            sj.check(sj.lib().spj_dev_memset(dO.ptr, 0xFF, nS * 8))
+            n, t = sj.join_device(dR, nR, dS, nS, dO, cfg, timed=True)
+            assert (n,) + sj.checksum_device(dO.ptr, n) == want, k
+            ts.append(t["total_ms"])
+        ct, rt = sj.graph_stats()
+        assert ct == c1 and rt == r1 and ts[0] > 0 and ts[1] > 0
+        run_and_check(sj, dR, nR, dS, nS, dO, cfg, want, 2, scratch)
+        c1, r1 = sj.graph_stats()
+        assert c1 == ct + 1 and r1 - rt == 2   # the timed call dropped the graph; both repeat it
         # another call in between drops the graph; the same call again is
         # launched directly, then captured anew
         dO2 = sj.DeviceArray(nS * 8)
