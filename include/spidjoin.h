@@ -70,8 +70,8 @@ typedef struct {
     int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
     int32_t radix_passes;     /* radix partition passes, 1 or 2 (single-instance calls only).  Two:
                                  the first pass partitions R and S by radix_bits of
-                                 SuperFastHash into contiguous partitions (heavy keys joined in its
-                                 S pass), the second splits every partition by the next bits
+                                 SuperFastHash (in the layout part_layout picks; heavy keys joined in
+                                 its S pass), the second splits every partition by the next bits
                                  (chosen per partition so R partitions fit one LDS table) and
                                  joins it.  AUTO = 2 when one pass of 12 bits would leave R
                                  partitions of 4 or more LDS tables (|R| > 3 * 2^26), else 1 */

@@ -319,9 +319,17 @@ static Context &second_pass_context()
 static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &err, PhaseTimes *times)
 {
     Context &c1 = default_context(), &c2 = second_pass_context();
+    // The first pass leaves a relation's partitions in cells (no histogram
+    // pass; the second pass's instances read a partition as its groups'
+    // segments) unless the caller fixed the layout or the segments would be
+    // short: a segment starts a scatter tile of its own, and segments of a
+    // few thousand tuples leave the second pass's rounds mostly empty (2^26
+    // x 2^26 in two passes, 2K-tuple segments: 2.65 ms against 2.23 counted).
+    constexpr uint64_t kMinSegment = 4096;
     InstSpec p1 = s0;
     p1.partition_only = true;
-    p1.r_contiguous = true;
+    p1.r_contiguous = ((uint64_t)s0.nR >> s0.cfg.bits) / kScatterGroups < kMinSegment;
+    p1.s_contiguous = ((uint64_t)s0.nS >> s0.cfg.bits) / kScatterGroups < kMinSegment;
     std::vector<uint32_t> cnt, errs;
     PhaseTimes t1, t2;
     hipError_t e = c1.run({p1}, cnt, errs, times ? &t1 : nullptr);
@@ -329,21 +337,36 @@ static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &er
     const uint32_t heavy_pairs = cnt[0];
     err = errs[0];
     const uint2 *rp = nullptr, *sp = nullptr;
-    std::vector<uint32_t> pr, ps;
-    e = c1.partitions(0, &rp, &sp, pr, ps);
+    uint32_t ngr = 0, ngs = 0;
+    std::vector<uint2> sr, ss;
+    e = c1.partition_segments(0, &rp, &sp, ngr, sr, ngs, ss);
     if (e != hipSuccess) return e;
     const uint32_t F1 = 1u << s0.cfg.bits;
     std::vector<InstSpec> sub;
-    std::vector<uint32_t> where;   // each instance's S partition start (its output offset)
+    std::vector<uint32_t> where;   // each instance's output offset: the S tuples of the partitions before it
+    uint64_t r_parted = 0, s_parted = 0;
     for (uint32_t p = 0; p < F1; p++) {
-        const uint32_t nr = pr[p + 1] - pr[p], ns = ps[p + 1] - ps[p];
-        if (!nr || !ns) continue;   // no pairs: an S partition without R tuples matches nothing
         InstSpec q;
-        q.R = rp + pr[p];
-        q.nR = nr;
-        q.S = sp + ps[p];
-        q.nS = ns;
-        q.out = s0.out + ps[p];
+        uint64_t nr = 0, ns = 0;
+        for (uint32_t k = 0; k < ngr; k++)
+            if (sr[(size_t)p * ngr + k].y) {
+                q.rseg.push_back(sr[(size_t)p * ngr + k]);
+                nr += sr[(size_t)p * ngr + k].y;
+            }
+        for (uint32_t k = 0; k < ngs; k++)
+            if (ss[(size_t)p * ngs + k].y) {
+                q.sseg.push_back(ss[(size_t)p * ngs + k]);
+                ns += ss[(size_t)p * ngs + k].y;
+            }
+        const uint64_t at = s_parted;
+        r_parted += nr;
+        s_parted += ns;
+        if (!nr || !ns) continue;   // no pairs: an S partition without R tuples matches nothing
+        q.R = rp;
+        q.nR = (uint32_t)nr;
+        q.S = sp;
+        q.nS = (uint32_t)ns;
+        q.out = s0.out + at;
         q.shift = s0.cfg.bits;
         Resolved c = default_params(nr, ns);
         c.bits = std::min(c.bits, kMaxDigitBits - s0.cfg.bits);
@@ -352,14 +375,19 @@ static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &er
         c.ipt = s0.cfg.ipt;
         c.smax = s0.cfg.smax;   // one value per batch
         // cells (no histogram pass) unless the caller fixed the layout: the
-        // first pass took the keys hot enough to overflow one
+        // first pass took the keys hot enough to overflow one.  Cells of
+        // twice the mean fill unless the caller sized them: a partition here
+        // is small (about 2K tuples a cell), so keys that the heavy
+        // detection leaves alone (Zipf 0.5 at 2^28: 8K occurrences of the
+        // hottest) still overflow cells of 1.5 mean + 8 sigma, and a redo
+        // of the whole batch counted costs more than the space.
         c.cells = s0.cfg.cells == 2 ? 1u : s0.cfg.cells;
-        c.cell_pct = s0.cfg.cell_pct;
+        c.cell_pct = s0.cfg.cell_pct ? s0.cfg.cell_pct : 200u;
         c.heavy = 0;   // taken by the first pass
         c.passes = 1;
         q.cfg = c;
-        sub.push_back(q);
-        where.push_back(ps[p]);
+        sub.push_back(std::move(q));
+        where.push_back((uint32_t)at);
     }
     uint64_t total = heavy_pairs;
     if (!sub.empty()) {
@@ -398,8 +426,8 @@ static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &er
         times->items = t2.items;
         times->join = t2.join;
         times->total = t1.total + t2.total;
-        times->r_parted = pr[F1];
-        times->s_parted = ps[F1];
+        times->r_parted = r_parted;
+        times->s_parted = s_parted;
         times->passes = 2;
     }
     return hipSuccess;

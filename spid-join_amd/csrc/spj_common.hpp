@@ -105,8 +105,9 @@ struct TileDesc {
     uint32_t cell0, cap;     // cell layout: the group's cells (cap = 0: counted layout)
 };
 
-// The tiles of one relation of one instance: ntiles tiles of `tile` tuples
-// (the last one shorter) at tile-list positions [first, first + ntiles),
+// The tiles of one relation of one instance, or of one segment of it: ntiles
+// tiles of `tile` tuples (the last one shorter) at tile-list positions
+// [first, first + ntiles),
 // in the scatter launch of tiles [launch0, launch0 + launch_n), whose
 // position decides a tile's scatter group.  The relation's group-0 counters
 // are at cnt_off0; cells = 1 + index of its first CellGroup (0: counted).
@@ -118,6 +119,7 @@ struct TileRun {
     uint64_t cnt_off0;
     uint32_t cells, bits;
     uint32_t launch0, launch_n;
+    uint32_t base, pad_;     // the run covers tuples [base, base + len) of the relation
 };
 
 // Scatter group of position p in a scatter launch: XCD x takes tiles x,

@@ -202,9 +202,14 @@ static bool same_specs(const std::vector<InstSpec> &a, const std::vector<InstSpe
     for (size_t i = 0; i < a.size(); i++) {
         const InstSpec &x = a[i], &y = b[i];
         if (x.R != y.R || x.nR != y.nR || x.S != y.S || x.nS != y.nS || x.out != y.out || x.Rp_pre != y.Rp_pre ||
-            x.pst_pre != y.pst_pre || x.r_contiguous != y.r_contiguous || x.shift != y.shift ||
-            x.partition_only != y.partition_only || !same_params(x.cfg, y.cfg))
+            x.pst_pre != y.pst_pre || x.r_contiguous != y.r_contiguous || x.s_contiguous != y.s_contiguous || x.shift != y.shift ||
+            x.partition_only != y.partition_only || !same_params(x.cfg, y.cfg) || x.rseg.size() != y.rseg.size() ||
+            x.sseg.size() != y.sseg.size())
             return false;
+        for (size_t k = 0; k < x.rseg.size(); k++)
+            if (x.rseg[k].x != y.rseg[k].x || x.rseg[k].y != y.rseg[k].y) return false;
+        for (size_t k = 0; k < x.sseg.size(); k++)
+            if (x.sseg[k].x != y.sseg[k].x || x.sseg[k].y != y.sseg[k].y) return false;
     }
     return true;
 }
@@ -310,18 +315,20 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         // for large hash-join relations; for S only with heavy-key handling,
         // which takes the keys frequent enough to overflow a cell (R keys
         // are unique).
-        const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj;
+        const bool segmented = !s.rseg.empty() || !s.sseg.empty();
+        if (segmented && (pre || nphj)) return hipErrorInvalidValue;
+        const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj && !segmented;
         auto use_cells = [&](uint32_t len, bool skew_safe) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
                    (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= (1u << 20) && skew_safe));
         };
-        // a first pass of two leaves contiguous partitions (the second
-        // pass's instances read them as plain relations)
+        // a first pass of two may leave its partitions in cells too: the
+        // second pass's instances read them as segments
         if (s.partition_only != specs[0].partition_only || (s.partition_only && (nphj || pre)))
             return hipErrorInvalidValue;
         if (s.shift + c.bits > kMaxDigitBits) return hipErrorInvalidValue;
-        cells[0][i] = !pre && !nphj && !s.r_contiguous && !s.partition_only && use_cells(s.nR, true);
-        cells[1][i] = !nphj && !s.partition_only && use_cells(s.nS, heavy_on);
+        cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true);
+        cells[1][i] = !nphj && !s.s_contiguous && use_cells(s.nS, heavy_on);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
@@ -350,7 +357,16 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             // counted: larger tiles, each costing F counter atomics in the
             // histogram pass
             const uint32_t tile = cl ? 2u * kScatterNT * c.ipt : c.tile;
-            const uint32_t T = (len + tile - 1) / tile;
+            uint32_t T = (len + tile - 1) / tile;
+            if (!(rel ? s.sseg : s.rseg).empty()) {
+                uint64_t tot = 0;
+                T = 0;
+                for (const uint2 &sg : rel ? s.sseg : s.rseg) {
+                    T += (sg.y + tile - 1) / tile;
+                    tot += sg.y;
+                }
+                if (tot != len) return hipErrorInvalidValue;   // the segments make up the relation
+            }
             uint64_t &cnt_total = cl ? cnt_cells : cnt_counted;   // cell offsets are shifted below
             cnt0[rel][i] = cnt_total;
             uint32_t seg_id = 0;
@@ -375,7 +391,19 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                 ru.len = len;
                 ru.cnt_off0 = cnt_total;
                 ru.bits = c.bits;
-                lists[rel == 0 ? (cl ? 0 : 1) : (cl ? 3 : 2)].push_back(ru);
+                auto &lst = lists[rel == 0 ? (cl ? 0 : 1) : (cl ? 3 : 2)];
+                const std::vector<uint2> &segv = rel ? s.sseg : s.rseg;
+                if (segv.empty()) {
+                    lst.push_back(ru);
+                } else {   // one run per segment, sharing the relation's counters and cells
+                    for (const uint2 &sg : segv) {
+                        if (!sg.y) continue;
+                        ru.base = sg.x;
+                        ru.len = sg.y;
+                        ru.ntiles = (sg.y + tile - 1) / tile;
+                        lst.push_back(ru);
+                    }
+                }
             }
             if (!cl) {
                 const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
@@ -700,19 +728,46 @@ hipError_t Context::compact(size_t i, uint32_t lo, uint32_t hi, uint32_t fill, c
     return hipStreamSynchronize(stream_);
 }
 
-hipError_t Context::partitions(size_t inst, const uint2 **rp, const uint2 **sp, std::vector<uint32_t> &pstart_r,
-                               std::vector<uint32_t> &pstart_s)
+hipError_t Context::partition_segments(size_t inst, const uint2 **rp, const uint2 **sp, uint32_t &ng_r,
+                                       std::vector<uint2> &segs_r, uint32_t &ng_s, std::vector<uint2> &segs_s)
 {
     if (inst >= h_insts_.size() || !partition_only_) return hipErrorInvalidValue;
     const InstDesc &d = h_insts_[inst];
-    const size_t F = size_t(1) << d.bits;
-    pstart_r.resize(F + 1);
-    pstart_s.resize(F + 1);
-    SPJ_TRY(hipMemcpyAsync(pstart_r.data(), pstart_.as<uint32_t>() + d.pstartR, (F + 1) * 4, hipMemcpyDeviceToHost,
-                           stream_));
-    SPJ_TRY(hipMemcpyAsync(pstart_s.data(), pstart_.as<uint32_t>() + d.pstartS, (F + 1) * 4, hipMemcpyDeviceToHost,
-                           stream_));
-    SPJ_TRY(hipStreamSynchronize(stream_));
+    const uint32_t F = 1u << d.bits;
+    for (uint32_t rel = 0; rel < 2; rel++) {
+        const uint32_t cl = rel ? d.s_cells : d.r_cells;
+        uint32_t &ng = rel ? ng_s : ng_r;
+        std::vector<uint2> &segs = rel ? segs_s : segs_r;
+        if (!cl) {   // counted: partition p is [pstart[p], pstart[p + 1])
+            std::vector<uint32_t> ps(F + 1);
+            SPJ_TRY(hipMemcpyAsync(ps.data(), pstart_.as<uint32_t>() + (rel ? d.pstartS : d.pstartR), (F + 1) * 4,
+                                   hipMemcpyDeviceToHost, stream_));
+            SPJ_TRY(hipStreamSynchronize(stream_));
+            ng = 1;
+            segs.resize(F);
+            for (uint32_t p = 0; p < F; p++) segs[p] = make_uint2(ps[p], ps[p + 1] - ps[p]);
+            continue;
+        }
+        // cells: group g's cell of digit p starts at cell0 + p cap; its cursor
+        // (counts[cnt_off + p], one array of F per group, the groups
+        // adjacent) ends where the group's tuples of p end
+        const CellGroup &g0 = h_cells_[cl - 1];
+        std::vector<uint32_t> cur((size_t)kScatterGroups * F);
+        SPJ_TRY(hipMemcpyAsync(cur.data(), counts_.as<uint32_t>() + g0.cnt_off, cur.size() * 4, hipMemcpyDeviceToHost,
+                               stream_));
+        SPJ_TRY(hipStreamSynchronize(stream_));
+        ng = kScatterGroups;
+        segs.resize((size_t)F * kScatterGroups);
+        for (uint32_t g = 0; g < kScatterGroups; g++) {
+            const CellGroup &cg = h_cells_[cl - 1 + g];
+            if (cg.cnt_off != g0.cnt_off + (uint64_t)g * F) return hipErrorInvalidValue;
+            for (uint32_t p = 0; p < F; p++) {
+                const uint32_t st = cg.cell0 + p * cg.cap, e = cur[(size_t)g * F + p];
+                if (e < st || e - st > cg.cap) return hipErrorInvalidValue;   // (an overflow redoes the batch counted)
+                segs[(size_t)p * kScatterGroups + g] = make_uint2(st, e - st);
+            }
+        }
+    }
     *rp = d.Rp;
     *sp = d.Sp;
     return hipSuccess;

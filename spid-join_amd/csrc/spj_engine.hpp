@@ -39,14 +39,19 @@ struct InstSpec {
     // partition starts.  Only S is partitioned then; heavy handling is off.
     const uint2 *Rp_pre = nullptr;
     const uint32_t *pst_pre = nullptr;
-    // R must end up in contiguous partitions (export_partitioned_r)
-    bool r_contiguous = false;
+    // R must end up in contiguous partitions (export_partitioned_r); S too
+    bool r_contiguous = false, s_contiguous = false;
     // digit = (SuperFastHash(key) >> shift) & (2^cfg.bits - 1): a second
     // radix pass over one first-pass partition takes the next bits
     uint32_t shift = 0;
-    // partition R and S only (counted layout, heavy pairs emitted to the
-    // back of out), no join: the first of two passes.  A batch is all or none.
+    // partition R and S only (heavy pairs emitted to the back of out), no
+    // join: the first of two passes.  A batch is all or none.
     bool partition_only = false;
+    // Optional: R (S) as segments (offset, length) of R (S), the relation
+    // being their concatenation and nR (nS) their total -- a first-pass
+    // partition that lies in cells.  No heavy keys and no no-partition join
+    // for such an instance.
+    std::vector<uint2> rseg, sseg;
 };
 
 // Milliseconds per phase (HIP events on the context's stream).
@@ -124,9 +129,12 @@ public:
 
     // After run()/collect() of a partition_only batch: instance i's
     // partitioned R and S (in this context's buffers, valid until its next
-    // run) and their F + 1 partition starts, read back to the host.
-    hipError_t partitions(size_t inst, const uint2 **rp, const uint2 **sp, std::vector<uint32_t> &pstart_r,
-                          std::vector<uint32_t> &pstart_s);
+    // run) as segments (offset into rp / sp, length), read back to the host:
+    // partition p of R is segs_r[p * ng_r + k], k < ng_r -- one segment per
+    // partition in the counted layout (ng = 1), one per scatter group in the
+    // cell layout (ng = 8).  The same for S.
+    hipError_t partition_segments(size_t inst, const uint2 **rp, const uint2 **sp, uint32_t &ng_r,
+                                  std::vector<uint2> &segs_r, uint32_t &ng_s, std::vector<uint2> &segs_s);
 
     // Make out[0, total) dense: copy segments (src, dst, len) of out, in
     // order, through a temporary buffer (sources and destinations may

@@ -266,8 +266,9 @@ __device__ __forceinline__ void expand_tile(const TileRun *__restrict__ runs, ui
     TileDesc td;
     td.inst = r.inst;
     td.rel = r.rel;
-    td.begin = (t - r.first) * r.tile;
-    td.end = (uint32_t)min((uint64_t)r.len, (uint64_t)td.begin + r.tile);
+    const uint32_t b = (t - r.first) * r.tile;
+    td.begin = r.base + b;
+    td.end = r.base + (uint32_t)min((uint64_t)r.len, (uint64_t)b + r.tile);
     td.cnt_off = r.cnt_off0 + ((uint64_t)g << r.bits);
     td.cell0 = r.cells ? cells[r.cells - 1 + g].cell0 : 0u;
     td.cap = r.cells ? cells[r.cells - 1 + g].cap : 0u;

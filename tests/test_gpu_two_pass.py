@@ -61,6 +61,23 @@ def test_two_pass_unmatched_zero_keys_and_holes(gpu, orc):
         assert t["passes"] == 2
 
 
+@pytest.mark.parametrize("layout,pct", [(1, 0), (1, 60), (0, 0)])
+def test_two_pass_first_pass_layouts(gpu, orc, layout, pct):
+    """The first pass in cells (its partitions read as segments by the
+    second), in cells too small (an overflow redoes it counted), and
+    counted: the same exact result."""
+    sj = gpu
+    R = sj.gen_r(1 << 20, 31)   # 32 partitions: 4K-tuple segments (cells only from 4K on)
+    S = sj.gen_s(1 << 21, R, 0.6, 32)
+    print(f"case two passes layout={layout} pct={pct}", flush=True)
+    cfg = sj.config(radix_passes=2, radix_bits=5, part_layout=layout, cell_capacity_pct=pct if pct else -1)
+    pairs, t = sj.join(R, S, cfg, timed=True)
+    check_equal(sj, orc, R, S, pairs)
+    assert t["passes"] == 2 and t["r_partitioned"] == len(R)
+    if layout == 1 and pct == 0:
+        assert t["hist_ms"] == 0.0   # no histogram pass in either pass
+
+
 def test_two_pass_duplicate_r_key_is_an_error(gpu):
     sj = gpu
     R = sj.gen_r(1 << 15, 61)
@@ -116,7 +133,7 @@ def test_two_pass_auto_at_2_28(gpu, orc):
     try:
         n, t = sj.join_device(dR, nR, dS, nS, dO, None, timed=True)
         got = (n,) + sj.checksum_device(dO.ptr, n)
-        assert t["passes"] == 2
+        assert t["passes"] == 2 and t["hist_ms"] == 0.0   # both passes in cells
         assert tuple(got) == tuple(want)
     finally:
         dR.free()
