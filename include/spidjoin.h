@@ -61,7 +61,9 @@ typedef struct {
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
     int64_t tile_tuples;      /* tuples per partition-pass tile of a counted relation (cells: two
                                  scatter rounds, 2 * 1024 * scatter_ipt) */
-    int64_t items_s_max;      /* S tuples per join work item */
+    int64_t items_s_max;      /* S tuples per join work item, at most: a call or batch whose S is
+                                 small uses smaller items, about two per CU (no fewer than 4096
+                                 S tuples each), so that its partitions are joined side by side */
     int32_t part_layout;      /* partitions of R and S: 0 counted (a histogram pass sizes them
                                  exactly), 1 cells (no histogram pass: fixed-capacity cells per
                                  scatter group and digit; the join is redone counted when a cell

@@ -290,8 +290,19 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     // relations' (set to the cell bases), so the two never overlap
     uint64_t cnt_counted = 0, cnt_cells = 0;
     uint64_t rp_total = 0, sp_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
-    // one S-range size per batch (the item kernel takes a single value)
-    const uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
+    uint32_t n_joined = 0;   // instances joined through partitions (not by the no-partition join)
+    // One S-range size per batch (the item kernel takes a single value):
+    // the configured one, but small enough for about two items per CU when
+    // the batch's S is small (each item builds its R table again, but the
+    // items run side by side: a lone 2^16 x 2^18 join has four partitions,
+    // and four items left k_join at one item's time on one CU)
+    uint32_t smax = specs.empty() ? 131072u : specs[0].cfg.smax;
+    {
+        uint64_t stot = 0;
+        for (const InstSpec &s : specs) stot += s.nS;
+        const uint64_t per = (stot + 2ull * device_info().cus - 1) / (2ull * std::max(1, device_info().cus));
+        smax = (uint32_t)std::min<uint64_t>(smax, std::max<uint64_t>(per, 4096));
+    }
     uint32_t nheavy_inst = 0;
     std::vector<uint64_t> rp_off(n), sp_off(n);
     std::vector<int> heavy_idx(n, -1);
@@ -349,6 +360,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             if (cells[0][i]) split = true;
         }
         items_bound += F + (uint64_t)s.nS / smax + 1;
+        if (!nphj) n_joined++;
         for (uint32_t rel = pre ? 1u : 0u; rel < 2; rel++) {
             const uint32_t len = nphj ? 0u : (rel ? s.nS : s.nR);
             const bool cl = cells[rel][i];
@@ -653,16 +665,19 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                         pstart_.as<uint32_t>(), max_bits_, st));
     if (scan_timed_) SPJ_TRY(hipEventRecord(ev_[3], st));
     // one persistent 1024-thread workgroup per CU (LDS-bound); R first
-    // when its tiles record heavy rids for the S tiles
-    const uint32_t sgrid = (uint32_t)std::max(1, device_info().cus);
+    // when its tiles record heavy rids for the S tiles.  (A launch of few
+    // tiles keeps the full grid too: each workgroup takes three tiles of its
+    // group's queue ahead, so a small grid leaves a few workgroups with all
+    // the tiles.)
+    const uint32_t cus = (uint32_t)std::max(1, device_info().cus);
     if (scat_split) {
         SPJ_TRY(launch_scatter(d_insts, d_tiles, scat_split, counts_.as<uint32_t>(), out_hi, err, ctrl + 4, max_bits_,
-                               max_ipt_, sgrid, st));
+                               max_ipt_, cus, st));
         if (timed) SPJ_TRY(hipEventRecord(ev_[7], st));
     }
     split_timed_ = timed && scat_split != 0;
     SPJ_TRY(launch_scatter(d_insts, d_tiles + scat_split, ntiles - scat_split, counts_.as<uint32_t>(), out_hi, err,
-                           ctrl + 4 + kScatterGroups, max_bits_, max_ipt_, sgrid, st));
+                           ctrl + 4 + kScatterGroups, max_bits_, max_ipt_, cus, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[4], st));
 
     // ---- join (none after a first pass of two)
@@ -677,14 +692,21 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         for (size_t i = 0; i < n; i++) h_ns_[i] = specs[i].nS;
         return hipSuccess;
     }
-    SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), d_cells, counts_.as<uint32_t>(),
-                              smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
+    // work items and the partitioned join, unless every instance takes the
+    // no-partition join (then neither has anything to do)
+    if (n_joined)
+        SPJ_TRY(launch_make_items(d_insts, (uint32_t)n, pstart_.as<uint32_t>(), d_cells, counts_.as<uint32_t>(),
+                                  smax, items_.as<WorkItem>(), ctrl, spart, rpart, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[5], st));
-    const DeviceInfo &di = device_info();
-    const uint32_t grid = (uint32_t)std::max(1, di.cus);   // one 1024-thread workgroup per CU (LDS-bound)
+    // one persistent 1024-thread workgroup per CU (LDS-bound), also for few
+    // items: a workgroup takes two items at its start, and a grid of as
+    // many workgroups as items left half of them idle (2^16 x 2^18: 4
+    // items, k_join 59.5 us with 256 workgroups, 109.8 us with 7)
+    const uint32_t grid = cus;
     if (record_ext_) SPJ_TRY(ext_.ensure(items_bound * sizeof(uint4)));
-    SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err, holes_.as<uint4>(), max_holes_,
-                        record_ext_ ? ext_.as<uint4>() : nullptr, grid, st));
+    if (n_joined)
+        SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err, holes_.as<uint4>(),
+                            max_holes_, record_ext_ ? ext_.as<uint4>() : nullptr, grid, st));
     SPJ_TRY(launch_nphj(d_insts, reinterpret_cast<const NphjJob *>(dd + o_nphj), (uint32_t)h_nphj_.size(),
                         nphj_.as<uint64_t>(), nphj_words, (uint32_t)nphj_bb, (uint32_t)nphj_pb, out_lo, out_fill,
                         err, st));
