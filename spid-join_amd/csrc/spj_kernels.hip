@@ -534,7 +534,12 @@ __global__ __launch_bounds__(kScatterNT) void k_scatter(const InstDesc *__restri
     uint32_t heavy_inst = DROP;   // instance whose heavy table is in LDS
     // software pipeline: the next round's tuples (possibly of the next tile)
     // are loaded while this round is scanned, staged and written
+    // positions past a round's end are not loaded: they keep an earlier
+    // round's tuple, or this empty one (the heavy lookups read the table for
+    // them too, and mask the result)
     uint2 nv[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) nv[j] = make_uint2(kEmptyKey, 0u);
     load_tile_round<IPT>(nv, insts, td, o);
     __builtin_amdgcn_s_waitcnt(0);   // entering the loop with no loads in flight (see the wait below)
     bool par = false;   // which counter array this round uses
