@@ -73,7 +73,8 @@ typedef struct {
     int32_t radix_passes;     /* radix partition passes, 1 or 2 (single-instance calls only).  Two:
                                  the first pass partitions R and S by radix_bits of
                                  SuperFastHash (in the layout part_layout picks; heavy keys joined in
-                                 its S pass), the second splits every partition by the next bits
+                                 its S pass; scatter_ipt applies to it), the second (with its own
+                                 round size) splits every partition by the next bits
                                  (chosen per partition so R partitions fit one LDS table) and
                                  joins it.  AUTO = 2 when one pass of 12 bits would leave R
                                  partitions of 4 or more LDS tables (|R| > 3 * 2^26), else 1 */

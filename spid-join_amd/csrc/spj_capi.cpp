@@ -342,6 +342,7 @@ static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &er
     e = c1.partition_segments(0, &rp, &sp, ngr, sr, ngs, ss);
     if (e != hipSuccess) return e;
     const uint32_t F1 = 1u << s0.cfg.bits;
+    const uint64_t seg_s = ((uint64_t)s0.nS >> s0.cfg.bits) / ngs;   // mean S segment
     std::vector<InstSpec> sub;
     std::vector<uint32_t> where;   // each instance's output offset: the S tuples of the partitions before it
     uint64_t r_parted = 0, s_parted = 0;
@@ -372,17 +373,25 @@ static hipError_t run_two_pass(const InstSpec &s0, uint32_t &count, uint32_t &er
         c.bits = std::min(c.bits, kMaxDigitBits - s0.cfg.bits);
         c.alg = s0.cfg.alg;
         c.table_log2 = s0.cfg.table_log2;
-        c.ipt = s0.cfg.ipt;
+        // rounds of its own size: a segment (a group's share of a
+        // first-pass partition) ends in a partly filled round, and the
+        // second pass's few digits make small rounds cheap.  Measured at
+        // 2^28 x 2^29 (16K-tuple S segments; 12 / 10 / 8 tuples per thread):
+        // 9.59 / 9.40 / 9.76 ms; at 2^28 x 2^30 (32K): 17.03 / 15.92 / 15.36.
+        c.ipt = seg_s >= 24576 ? 8u : 10u;
         c.smax = s0.cfg.smax;   // one value per batch
         // cells (no histogram pass) unless the caller fixed the layout: the
         // first pass took the keys hot enough to overflow one.  Cells of
-        // twice the mean fill unless the caller sized them: a partition here
-        // is small (about 2K tuples a cell), so keys that the heavy
-        // detection leaves alone (Zipf 0.5 at 2^28: 8K occurrences of the
-        // hottest) still overflow cells of 1.5 mean + 8 sigma, and a redo
-        // of the whole batch counted costs more than the space.
+        // twice the mean fill (plus the usual 8 sigma + 64) unless the caller
+        // sized them: a partition here is small (about 2K tuples a cell), so
+        // keys that the heavy detection leaves alone (Zipf 0.5 at 2^28: 8K
+        // occurrences of the hottest) still overflow cells of 1.5 mean +
+        // 8 sigma, and a redo of the whole batch counted costs more than the
+        // space.  (Twice the mean alone is too tight for a group that gets
+        // only a short segment's tail.)
         c.cells = s0.cfg.cells == 2 ? 1u : s0.cfg.cells;
-        c.cell_pct = s0.cfg.cell_pct ? s0.cfg.cell_pct : 200u;
+        c.cell_pct = s0.cfg.cell_pct;
+        c.cell_mean_x2 = 1;
         c.heavy = 0;   // taken by the first pass
         c.passes = 1;
         q.cfg = c;

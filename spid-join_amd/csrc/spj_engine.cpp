@@ -178,22 +178,23 @@ static inline size_t ctrl_words(size_t n) { return kCtrlHead + kCtrlArrays * n; 
 static inline size_t ctrl_at(size_t n, int which, size_t i) { return kCtrlHead + (size_t)which * n + i; }
 
 // Capacity of one S cell that takes a group's tuples of one digit, from the
-// group's tuple count n and the fan-out F: by default 1.5 times the mean
-// fill plus 8 standard deviations of a uniform spread plus 64, so that only
-// keys too frequent for the heavy detection to have missed overflow a cell.
-static uint32_t cell_capacity(uint64_t n, uint32_t F, uint32_t pct)
+// group's tuple count n and the fan-out F: by default 1.5 (x2: 2) times the
+// mean fill plus 8 standard deviations of a uniform spread plus 64, so that
+// only keys too frequent for the heavy detection to have missed overflow a
+// cell.
+static uint32_t cell_capacity(uint64_t n, uint32_t F, uint32_t pct, bool x2)
 {
     if (!n) return 0;
     const double m = (double)n / F;
     if (pct) return (uint32_t)std::ceil(m * pct / 100.0) + 1u;
-    return (uint32_t)std::ceil(1.5 * m + 8.0 * std::sqrt(m) + 64.0);
+    return (uint32_t)std::ceil((x2 ? 2.0 : 1.5) * m + 8.0 * std::sqrt(m) + 64.0);
 }
 
 static bool same_params(const Resolved &a, const Resolved &b)
 {
     return a.bits == b.bits && a.table_log2 == b.table_log2 && a.alg == b.alg && a.heavy == b.heavy &&
            a.heavy_min == b.heavy_min && a.ipt == b.ipt && a.tile == b.tile && a.smax == b.smax && a.cells == b.cells &&
-           a.cell_pct == b.cell_pct && a.passes == b.passes;
+           a.cell_pct == b.cell_pct && a.cell_mean_x2 == b.cell_mean_x2 && a.passes == b.passes;
 }
 
 static bool same_specs(const std::vector<InstSpec> &a, const std::vector<InstSpec> &b)
@@ -486,7 +487,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             const uint32_t first = (uint32_t)h_cells_.size() + 1;
             for (uint32_t g = 0; g < kScatterGroups; g++) {
                 const size_t k = gk(i, rel, g);
-                gcap[k] = cell_capacity(gsize[k], F, specs[i].cfg.cell_pct);
+                gcap[k] = cell_capacity(gsize[k], F, specs[i].cfg.cell_pct, specs[i].cfg.cell_mean_x2 != 0);
                 gcell0[k] = (uint32_t)region;
                 region += (uint64_t)F * gcap[k];
                 CellGroup cg{};

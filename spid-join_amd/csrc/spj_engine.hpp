@@ -24,6 +24,7 @@ struct Resolved {
     uint32_t smax = 131072;   // S tuples per work item
     uint32_t cells = 2;       // S layout: 0 counted (histogram pass), 1 cells (none), 2 auto
     uint32_t cell_pct = 0;    // cell capacity in % of the mean cell fill (0 = automatic)
+    uint32_t cell_mean_x2 = 0;// automatic capacity from twice the mean fill (a second radix pass)
     uint32_t passes = 1;      // radix partition passes: 1, or 2 (bits is the first pass's)
 };
 
