@@ -148,27 +148,16 @@ struct PrepArgs {
 };
 
 // One (instance, relation) whose group counts are scanned together: t_count
-// groups (kScatterGroups), group g's counters at cnt_off0 + g * F; chunk
-// partials live at part_off; partition starts at pstart_off (F + 1 entries).
+// groups (kScatterGroups), group g's counters at cnt_off0 + g * F;
+// partition starts at pstart_off (F + 1 entries).
 struct ScanSeg {
     uint32_t inst;
     uint32_t rel;
     uint32_t bits;
     uint32_t t_count;
     uint64_t cnt_off0;
-    uint64_t part_off;
     uint64_t pstart_off;
 };
-
-// One block of a column-scan phase: segment, 256-digit chunk, group chunk.
-struct ScanBlock {
-    uint32_t seg;
-    uint32_t dchunk;
-    uint32_t tchunk;
-    uint32_t pad;
-};
-
-constexpr uint32_t kScanTileChunk = 32;   // tiles per column-scan chunk
 
 // Heavy-key detection of one instance: nblocks sample blocks, every
 // stride-th 8K-tuple block of S.  Phase 0 samples every kPreSample-th of

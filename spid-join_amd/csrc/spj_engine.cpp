@@ -157,7 +157,7 @@ Context::~Context()
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (desc_ev_) (void)hipEventDestroy(desc_ev_);
-    DevBuf *bufs[] = {&desc_, &counts_, &part_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
+    DevBuf *bufs[] = {&desc_, &counts_, &pstart_, &rp_, &sp_, &items_, &ctrl_,
                       &heavy_, &cand_, &holes_, &tmp_, &segbuf_, &tiles_, &nphj_, &ext_};
     for (auto *b : bufs) b->release();
     pin_desc_.release();
@@ -275,8 +275,6 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     h_insts_.assign(n, InstDesc{});
     h_cells_.clear();
     h_segs_.clear();
-    h_ad_.clear();
-    h_b_.clear();
     max_bits_ = 0;
     max_table_ = 0;
     max_ipt_ = 4;
@@ -290,7 +288,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     // counters: the counted relations' first (zeroed), then the cell
     // relations' (set to the cell bases), so the two never overlap
     uint64_t cnt_counted = 0, cnt_cells = 0;
-    uint64_t rp_total = 0, sp_total = 0, part_total = 0, pst_total = 0, items_bound = 0;
+    uint64_t rp_total = 0, sp_total = 0, pst_total = 0, items_bound = 0;
     uint32_t n_joined = 0;   // instances joined through partitions (not by the no-partition join)
     // One S-range size per batch (the item kernel takes a single value):
     // the configured one, but small enough for about two items per CU when
@@ -382,7 +380,6 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             }
             uint64_t &cnt_total = cl ? cnt_cells : cnt_counted;   // cell offsets are shifted below
             cnt0[rel][i] = cnt_total;
-            uint32_t seg_id = 0;
             if (!cl) {
                 ScanSeg sg{};
                 sg.inst = (uint32_t)i;
@@ -390,9 +387,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                 sg.bits = c.bits;
                 sg.t_count = kScatterGroups;
                 sg.cnt_off0 = cnt_total;
-                sg.part_off = part_total;
                 sg.pstart_off = rel ? d.pstartS : d.pstartR;
-                seg_id = (uint32_t)h_segs_.size();
                 h_segs_.push_back(sg);
             }
             if (T) {
@@ -417,14 +412,6 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                         lst.push_back(ru);
                     }
                 }
-            }
-            if (!cl) {
-                const uint32_t dch = (F + 255) / 256, tch = (kScatterGroups + kScanTileChunk - 1) / kScanTileChunk;
-                for (uint32_t dc = 0; dc < dch; dc++) {
-                    h_b_.push_back(ScanBlock{seg_id, dc, 0, 0});
-                    for (uint32_t tc = 0; tc < tch; tc++) h_ad_.push_back(ScanBlock{seg_id, dc, tc, 0});
-                }
-                part_total += (uint64_t)tch * F;
             }
             cnt_total += (uint64_t)kScatterGroups * F;
         }
@@ -549,15 +536,12 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     const size_t o_insts = 0;
     const size_t o_runs = o_insts + al(n * sizeof(InstDesc));
     const size_t o_segs = o_runs + al(h_runs_.size() * sizeof(TileRun));
-    const size_t o_ad = o_segs + al(h_segs_.size() * sizeof(ScanSeg));
-    const size_t o_b = o_ad + al(h_ad_.size() * sizeof(ScanBlock));
-    const size_t o_jobs = o_b + al(h_b_.size() * sizeof(ScanBlock));
+    const size_t o_jobs = o_segs + al(h_segs_.size() * sizeof(ScanSeg));
     const size_t o_cells = o_jobs + al(h_jobs_.size() * sizeof(HeavyJob));
     const size_t o_nphj = o_cells + al(h_cells_.size() * sizeof(CellGroup));
     const size_t desc_bytes = o_nphj + al(h_nphj_.size() * sizeof(NphjJob));
     SPJ_TRY(desc_.ensure(desc_bytes));
     SPJ_TRY(counts_.ensure(cnt_total * 4));
-    SPJ_TRY(part_.ensure(part_total * 4));
     SPJ_TRY(pstart_.ensure(pst_total * 4));
     SPJ_TRY(rp_.ensure(rp_total * 8));
     SPJ_TRY(sp_.ensure(sp_total * 8));
@@ -595,8 +579,6 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             if (n) memcpy(h + o_insts, h_insts_.data(), n * sizeof(InstDesc));
             if (!h_runs_.empty()) memcpy(h + o_runs, h_runs_.data(), h_runs_.size() * sizeof(TileRun));
             if (!h_segs_.empty()) memcpy(h + o_segs, h_segs_.data(), h_segs_.size() * sizeof(ScanSeg));
-            if (!h_ad_.empty()) memcpy(h + o_ad, h_ad_.data(), h_ad_.size() * sizeof(ScanBlock));
-            if (!h_b_.empty()) memcpy(h + o_b, h_b_.data(), h_b_.size() * sizeof(ScanBlock));
             if (!h_jobs_.empty()) memcpy(h + o_jobs, h_jobs_.data(), h_jobs_.size() * sizeof(HeavyJob));
             if (!h_cells_.empty()) memcpy(h + o_cells, h_cells_.data(), h_cells_.size() * sizeof(CellGroup));
             if (!h_nphj_.empty()) memcpy(h + o_nphj, h_nphj_.data(), h_nphj_.size() * sizeof(NphjJob));
@@ -609,8 +591,6 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     const TileRun *d_runs = reinterpret_cast<const TileRun *>(dd + o_runs);
     const TileDesc *d_tiles = tiles_.as<TileDesc>();
     const ScanSeg *d_segs = reinterpret_cast<const ScanSeg *>(dd + o_segs);
-    const ScanBlock *d_ad = reinterpret_cast<const ScanBlock *>(dd + o_ad);
-    const ScanBlock *d_b = reinterpret_cast<const ScanBlock *>(dd + o_b);
     const HeavyJob *d_jobs = reinterpret_cast<const HeavyJob *>(dd + o_jobs);
     const CellGroup *d_cells = reinterpret_cast<const CellGroup *>(dd + o_cells);
     for (size_t i = 0; i < n; i++)   // resident tables: their partition starts
@@ -661,9 +641,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     hist_timed_ = timed && n_counted != 0;
     scan_timed_ = timed && !h_segs_.empty();
     if (hist_timed_) SPJ_TRY(hipEventRecord(ev_[2], st));
-    SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), d_ad, (uint32_t)h_ad_.size(), d_b, (uint32_t)h_b_.size(),
-                        counts_.as<uint32_t>(), part_.as<uint32_t>(),
-                        pstart_.as<uint32_t>(), max_bits_, st));
+    SPJ_TRY(launch_scan(d_segs, (uint32_t)h_segs_.size(), counts_.as<uint32_t>(), pstart_.as<uint32_t>(), max_bits_, st));
     if (scan_timed_) SPJ_TRY(hipEventRecord(ev_[3], st));
     // one persistent 1024-thread workgroup per CU (LDS-bound); R first
     // when its tiles record heavy rids for the S tiles.  (A launch of few

@@ -159,7 +159,7 @@ private:
     bool scan_timed_ = false;   // ... ev_[3] (it launched a scan)
     // desc_: every descriptor array of a batch (instances, tiles, scan
     // segments and blocks, heavy jobs) in one block, uploaded by one copy
-    DevBuf desc_, counts_, part_, pstart_;
+    DevBuf desc_, counts_, pstart_;
     DevBuf rp_, sp_, items_, ctrl_, heavy_, cand_, holes_, tmp_, segbuf_, tiles_, nphj_, ext_;
     bool record_ext_ = false;
     PinBuf pin_desc_, pin_ctrl_;
@@ -180,7 +180,6 @@ private:
     std::vector<InstSpec> h_specs_;
     bool force_counted_ = false;
     std::vector<ScanSeg> h_segs_;
-    std::vector<ScanBlock> h_ad_, h_b_;
     std::vector<uint32_t> h_ctrl_;
     uint32_t max_bits_ = 0, max_table_ = 0;
     int max_ipt_ = 8;

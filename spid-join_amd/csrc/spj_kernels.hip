@@ -2,7 +2,7 @@
 //
 // Phase map (reference operator -> kernel):
 //   GLB_PARTITION_COUNT (glb_partition_count.c)          -> k_hist
-//   CALCULATE_PAGE_HISTOGRAM (queryHostFunc.cc:875-960)   -> k_scan_a/b/c/d
+//   CALCULATE_PAGE_HISTOGRAM (queryHostFunc.cc:875-960)   -> k_scan
 //   GLB_PARTITION_PACKET + ROTATE_AND_STREAM              -> k_scatter
 //   BG_BROADCAST / CHIPWISE_BROADCAST (SPID replication)  -> k_heavy_sample,
 //        k_heavy_select, heavy capture in k_hist (R) and
@@ -354,80 +354,41 @@ __global__ __launch_bounds__(1024) void k_hist(const InstDesc *__restrict__ inst
 }
 
 // ---------------------------------------------------------------------------
-// Column scan of group counts into global write offsets
-// (the page histogram + cross-rank prefix of queryHostFunc.cc:329-368,
-// 1574-1769, done on the device and without max-padding).  A segment's
-// "tiles" here are its scatter groups; the results are the groups' cursor
-// start offsets.
-//   a: per group chunk, exclusive prefix down each digit column; chunk sums
-//   b: per digit, exclusive prefix over chunk sums; digit totals
-//   c: per segment, exclusive prefix over digits -> partition starts
-//   d: add chunk base and partition start to every group offset
+// Column scan of group counts into global write offsets (the page histogram
+// + cross-rank prefix of queryHostFunc.cc:329-368, 1574-1769, done on the
+// device and without max-padding), one workgroup per segment (a relation of
+// an instance): for each digit, an exclusive prefix down its column of
+// t_count group counts (the groups' offsets inside the partition), the
+// column totals scanned over the digits (the partition starts), and the
+// partition start added to each group offset -- which become the scatter's
+// cursors.  One launch: the segment has only kScatterGroups columns.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scan_a(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
-                                                uint32_t *__restrict__ counts, uint32_t *__restrict__ part)
-{
-    const ScanBlock bk = blocks[blockIdx.x];
-    const ScanSeg sg = segs[bk.seg];
-    const uint32_t F = 1u << sg.bits;
-    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
-    if (d >= F) return;
-    const uint32_t t0 = bk.tchunk * kScanTileChunk, t1 = min(sg.t_count, t0 + kScanTileChunk);
-    uint32_t run = 0;
-    for (uint32_t t = t0; t < t1; t++) {
-        uint32_t *c = counts + sg.cnt_off0 + (uint64_t)t * F + d;
-        uint32_t v = *c;
-        *c = run;
-        run += v;
-    }
-    part[sg.part_off + (uint64_t)bk.tchunk * F + d] = run;
-}
-
-__global__ __launch_bounds__(256) void k_scan_b(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
-                                                uint32_t *__restrict__ part, uint32_t *__restrict__ pstart)
-{
-    const ScanBlock bk = blocks[blockIdx.x];
-    const ScanSeg sg = segs[bk.seg];
-    const uint32_t F = 1u << sg.bits;
-    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
-    if (d >= F) return;
-    const uint32_t nch = (sg.t_count + kScanTileChunk - 1) / kScanTileChunk;
-    uint32_t run = 0;
-    for (uint32_t c = 0; c < nch; c++) {
-        uint32_t *p = part + sg.part_off + (uint64_t)c * F + d;
-        uint32_t v = *p;
-        *p = run;
-        run += v;
-    }
-    pstart[sg.pstart_off + d] = run;   // digit total; turned into starts by k_scan_c
-}
-
-__global__ __launch_bounds__(1024) void k_scan_c(const ScanSeg *__restrict__ segs, uint32_t *__restrict__ pstart)
+__global__ __launch_bounds__(1024) void k_scan(const ScanSeg *__restrict__ segs, uint32_t *__restrict__ counts,
+                                               uint32_t *__restrict__ pstart)
 {
     __shared__ uint32_t wtmp[1024 / kWave + 1];
-    extern __shared__ __align__(16) uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];   // F digit totals, then starts
     const ScanSeg sg = segs[blockIdx.x];
     const uint32_t F = 1u << sg.bits;
-    uint32_t *p = pstart + sg.pstart_off;
-    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) lds[i] = p[i];
+    uint32_t *c = counts + sg.cnt_off0;   // group g, digit d at c[g F + d]
+    for (uint32_t d = threadIdx.x; d < F; d += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t g = 0; g < sg.t_count; g++) {
+            const uint32_t v = c[(uint64_t)g * F + d];
+            c[(uint64_t)g * F + d] = run;
+            run += v;
+        }
+        lds[d] = run;
+    }
     __syncthreads();
-    uint32_t total = block_exclusive_scan(lds, F, wtmp);
-    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) p[i] = lds[i];
+    const uint32_t total = block_exclusive_scan(lds, F, wtmp);   // lds[d] = start of partition d
+    uint32_t *p = pstart + sg.pstart_off;
+    for (uint32_t d = threadIdx.x; d < F; d += blockDim.x) {
+        const uint32_t base = lds[d];
+        for (uint32_t g = 0; g < sg.t_count; g++) c[(uint64_t)g * F + d] += base;
+        p[d] = base;
+    }
     if (threadIdx.x == 0) p[F] = total;
-}
-
-__global__ __launch_bounds__(256) void k_scan_d(const ScanSeg *__restrict__ segs, const ScanBlock *__restrict__ blocks,
-                                                uint32_t *__restrict__ counts, const uint32_t *__restrict__ part,
-                                                const uint32_t *__restrict__ pstart)
-{
-    const ScanBlock bk = blocks[blockIdx.x];
-    const ScanSeg sg = segs[bk.seg];
-    const uint32_t F = 1u << sg.bits;
-    const uint32_t d = bk.dchunk * 256u + threadIdx.x;
-    if (d >= F) return;
-    const uint32_t base = part[sg.part_off + (uint64_t)bk.tchunk * F + d] + pstart[sg.pstart_off + d];
-    const uint32_t t0 = bk.tchunk * kScanTileChunk, t1 = min(sg.t_count, t0 + kScanTileChunk);
-    for (uint32_t t = t0; t < t1; t++) counts[sg.cnt_off0 + (uint64_t)t * F + d] += base;
 }
 
 // ---------------------------------------------------------------------------
@@ -2330,18 +2291,14 @@ hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t nt
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk_ad, uint32_t nblk_ad,
-                       const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
-                       uint32_t max_bits, hipStream_t st)
+hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, uint32_t *counts, uint32_t *pstart, uint32_t max_bits,
+                       hipStream_t st)
 {
     if (!nsegs) return hipSuccess;
-    hipLaunchKernelGGL(k_scan_a, dim3(nblk_ad), dim3(256), 0, st, segs, blk_ad, counts, part);
-    hipLaunchKernelGGL(k_scan_b, dim3(nblk_b), dim3(256), 0, st, segs, blk_b, part, pstart);
     size_t l = (size_t(1) << max_bits) * 4;
-    hipError_t e = set_lds((const void *)k_scan_c, l);
+    hipError_t e = set_lds((const void *)k_scan, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_c, dim3(nsegs), dim3(1024), l, st, segs, pstart);
-    hipLaunchKernelGGL(k_scan_d, dim3(nblk_ad), dim3(256), 0, st, segs, blk_ad, counts, part, pstart);
+    hipLaunchKernelGGL(k_scan, dim3(nsegs), dim3(1024), l, st, segs, counts, pstart);
     return hipGetLastError();
 }
 

@@ -16,9 +16,8 @@ uint32_t cand_words();   // words of one job's candidate map (keys, counts, foun
 
 hipError_t launch_hist(const InstDesc *insts, const TileDesc *tiles, uint32_t ntiles, uint32_t *counts,
                        uint32_t max_bits, hipStream_t st);
-hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, const ScanBlock *blk_ad, uint32_t nblk_ad,
-                       const ScanBlock *blk_b, uint32_t nblk_b, uint32_t *counts, uint32_t *part, uint32_t *pstart,
-                       uint32_t max_bits, hipStream_t st);
+hipError_t launch_scan(const ScanSeg *segs, uint32_t nsegs, uint32_t *counts, uint32_t *pstart, uint32_t max_bits,
+                       hipStream_t st);
 // Persistent scatter over `grid` workgroups (rounded up to a multiple of 8);
 // counts holds the group cursors (scanned, or cell bases) and is advanced in
 // place; tq holds 8 zeroed tile-queue counters (one per XCD).
