@@ -67,8 +67,8 @@ typedef struct {
     int32_t part_layout;      /* partitions of R and S: 0 counted (a histogram pass sizes them
                                  exactly), 1 cells (no histogram pass: fixed-capacity cells per
                                  scatter group and digit; the join is redone counted when a cell
-                                 overflows); AUTO = cells for a hash or sort-merge relation of >= 2^20 tuples
-                                 (for S only with heavy-key handling on) */
+                                 overflows); AUTO = cells for R of a hash or sort-merge join, and for its S
+                                 from 2^20 tuples with heavy-key handling on */
     int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
     int32_t radix_passes;     /* radix partition passes, 1 or 2 (single-instance calls only).  Two:
                                  the first pass partitions R and S by radix_bits of

@@ -328,17 +328,22 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         const bool segmented = !s.rseg.empty() || !s.sseg.empty();
         if (segmented && (pre || nphj)) return hipErrorInvalidValue;
         const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj && !segmented;
-        auto use_cells = [&](uint32_t len, bool skew_safe) {
+        // AUTO: cells for R of any size (unique keys never crowd a cell; a
+        // batch of 4096 mixed joins, all R below 2^20: 21.2 -> 20.1 ms
+        // without their histogram pass), for S from 2^20 tuples and only
+        // with heavy-key handling (one small skewed S overflowing a cell
+        // would redo its whole batch counted)
+        auto use_cells = [&](uint32_t len, bool skew_safe, uint32_t min_len) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
-                   (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= (1u << 20) && skew_safe));
+                   (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= min_len && skew_safe));
         };
         // a first pass of two may leave its partitions in cells too: the
         // second pass's instances read them as segments
         if (s.partition_only != specs[0].partition_only || (s.partition_only && (nphj || pre)))
             return hipErrorInvalidValue;
         if (s.shift + c.bits > kMaxDigitBits) return hipErrorInvalidValue;
-        cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true);
-        cells[1][i] = !nphj && !s.s_contiguous && use_cells(s.nS, heavy_on);
+        cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true, 1u);
+        cells[1][i] = !nphj && !s.s_contiguous && use_cells(s.nS, heavy_on, 1u << 20);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;
