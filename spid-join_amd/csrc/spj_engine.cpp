@@ -511,7 +511,17 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         j.min_count = specs[i].cfg.heavy_min;
         j.cand_off = (uint64_t)heavy_idx[i] * cand_words();
         hblocks[0] += pre;
-        hblocks[1] += j.nblocks - pre;
+        // The second sample only where the pre-sample can miss a key that
+        // overflows a cell.  Such a key holds more than half a cell's mean
+        // fill in each of the 8 groups: a share of S above 1 / (2 F).  The
+        // pre-sample (kLocalMin in an 8192-key block) sees shares from
+        // about 1 / 2048 on, so with F < 1024 partitions it finds every one;
+        // the second sample then only adds milder keys, whose direct
+        // emission did not pay for the sampling (4096 mixed joins, |R| <=
+        // 2^20: 20.4 -> 19.5 ms without it).  At 2^26 x 2^28, Zipf 1.0
+        // (F = 4096) the pre-sample alone left a key that overflowed a cell
+        // (5.2 ms with the counted redo, 2.36 ms with the second sample).
+        if (specs[i].cfg.bits >= 10) hblocks[1] += j.nblocks - pre;
         h_jobs_.push_back(j);
     }
 
