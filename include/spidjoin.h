@@ -56,7 +56,7 @@ typedef struct {
                                  >= 2|R| 8-byte slots in device memory, probed by every S tuple */
     int32_t radix_bits;       /* partition-pass radix bits (0..12) */
     int32_t table_log2;       /* LDS hash-table buckets, log2 (6..13; 12 = the reference's 4096) */
-    int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on */
+    int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on; AUTO = on for |S| >= 2^14 */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
     int64_t tile_tuples;      /* tuples per partition-pass tile of a counted relation (cells: two
@@ -68,7 +68,7 @@ typedef struct {
                                  exactly), 1 cells (no histogram pass: fixed-capacity cells per
                                  scatter group and digit; the join is redone counted when a cell
                                  overflows); AUTO = cells for R of a hash or sort-merge join, and for its S
-                                 from 2^20 tuples with heavy-key handling on */
+                                 from 2^14 tuples with heavy-key handling on */
     int32_t cell_capacity_pct;/* cells: capacity in % of the mean fill (1..200); AUTO = 150% + 8 sigma + 64 */
     int32_t radix_passes;     /* radix partition passes, 1 or 2 (single-instance calls only).  Two:
                                  the first pass partitions R and S by radix_bits of

@@ -122,7 +122,7 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     r.ipt = 12;
     r.tile = default_tile(std::max(nR, nS));
     r.smax = 131072;   // at 16K R tuples per partition: several S tuples per table build
-    r.heavy = nS >= (1u << 20) ? 1u : 0u;
+    r.heavy = nS >= (1u << 14) ? 1u : 0u;   // (for S's cells: see use_cells)
     r.heavy_min = 16;
     r.alg = 0;
     // Two passes when one pass of kMaxBits would leave R partitions of four
@@ -330,9 +330,11 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         const bool heavy_on = c.heavy && s.nS > 0 && !pre && !nphj && !segmented;
         // AUTO: cells for R of any size (unique keys never crowd a cell; a
         // batch of 4096 mixed joins, all R below 2^20: 21.2 -> 20.1 ms
-        // without their histogram pass), for S from 2^20 tuples and only
-        // with heavy-key handling (one small skewed S overflowing a cell
-        // would redo its whole batch counted)
+        // without their histogram pass), for S from 2^14 tuples and only
+        // with heavy-key handling, whose pre-sample takes the keys that
+        // could overflow a cell (one skewed S overflowing a cell would redo
+        // its whole batch counted).  Heavy detection and cells for S from
+        // 2^14 instead of 2^20 tuples: the same batch 19.9 -> 19.1 ms.
         auto use_cells = [&](uint32_t len, bool skew_safe, uint32_t min_len) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
                    (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= min_len && skew_safe));
@@ -343,7 +345,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
             return hipErrorInvalidValue;
         if (s.shift + c.bits > kMaxDigitBits) return hipErrorInvalidValue;
         cells[0][i] = !pre && !nphj && !s.r_contiguous && use_cells(s.nR, true, 1u);
-        cells[1][i] = !nphj && !s.s_contiguous && use_cells(s.nS, heavy_on, 1u << 20);
+        cells[1][i] = !nphj && !s.s_contiguous && use_cells(s.nS, heavy_on, 1u << 14);
         InstDesc &d = h_insts_[i];
         d.R = s.R;
         d.S = s.S;

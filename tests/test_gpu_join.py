@@ -465,8 +465,8 @@ def test_join_instance_plan_errors_and_nphj(gpu, orc):
     import test_plan
 
     sj = gpu
-    R = sj.gen_r(1 << 15, 91)
-    S = sj.gen_s(1 << 17, R, 0.9, 92)
+    R = sj.gen_r(1 << 12, 91)
+    S = sj.gen_s(1 << 13, R, 0.9, 92)   # S below 2^14: no heavy keys, so R and S scatter in one launch
     ji = sj.JoinInstance(16, 1, 16)
     ji.load_column(R, "left")
     ji.load_column(S, "right")
