@@ -56,7 +56,8 @@ typedef struct {
                                  >= 2|R| 8-byte slots in device memory, probed by every S tuple */
     int32_t radix_bits;       /* partition-pass radix bits (0..12) */
     int32_t table_log2;       /* LDS hash-table buckets, log2 (6..13; 12 = the reference's 4096) */
-    int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on; AUTO = on for |S| >= 2^14 */
+    int32_t heavy;            /* heavy-key skew handling: 0 off, 1 on; AUTO = on for |S| >= 2^20 (in a
+                                 batch: >= 2^14) */
     int32_t heavy_min_count;  /* summed sample count that makes a key heavy */
     int32_t scatter_ipt;      /* scatter tuples per thread per round (1024 threads): 4, 8, 10 or 12 */
     int64_t tile_tuples;      /* tuples per partition-pass tile of a counted relation (cells: two

@@ -145,6 +145,8 @@ struct PrepArgs {
     const CellGroup *cells;
     uint32_t ncells;
     uint32_t pad;
+    unsigned long long *nphj;   // the no-partition joins' tables: nphj[0, nnphj) cleared
+    uint64_t nnphj;
 };
 
 // One (instance, relation) whose group counts are scanned together: t_count

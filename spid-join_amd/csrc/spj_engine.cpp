@@ -122,7 +122,7 @@ Resolved default_params(uint32_t nR, uint32_t nS)
     r.ipt = 12;
     r.tile = default_tile(std::max(nR, nS));
     r.smax = 131072;   // at 16K R tuples per partition: several S tuples per table build
-    r.heavy = nS >= (1u << 14) ? 1u : 0u;   // (for S's cells: see use_cells)
+    r.heavy = nS >= (1u << 20) ? 1u : 0u;   // (a batch lowers this to 2^14; see spj_join_batch_device)
     r.heavy_min = 16;
     r.alg = 0;
     // Two passes when one pass of kMaxBits would leave R partitions of four
@@ -333,8 +333,8 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         // without their histogram pass), for S from 2^14 tuples and only
         // with heavy-key handling, whose pre-sample takes the keys that
         // could overflow a cell (one skewed S overflowing a cell would redo
-        // its whole batch counted).  Heavy detection and cells for S from
-        // 2^14 instead of 2^20 tuples: the same batch 19.9 -> 19.1 ms.
+        // its whole batch counted).  AUTO turns heavy handling on from 2^20
+        // S tuples in single calls, from 2^14 in batches.
         auto use_cells = [&](uint32_t len, bool skew_safe, uint32_t min_len) {
             return !force_counted_ && len > 0 && len <= (1u << 30) && c.cell_pct <= 200 &&
                    (c.cells == 1 || (c.cells == 2 && (c.alg == 0 || c.alg == 2) && len >= min_len && skew_safe));
@@ -641,6 +641,8 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
         pa.tiles = tiles_.as<TileDesc>();
         pa.cells = d_cells;
         pa.ncells = (uint32_t)h_cells_.size();
+        pa.nphj = nphj_words ? nphj_.as<unsigned long long>() : nullptr;   // cleared here: one launch fewer
+        pa.nnphj = nphj_words;
         SPJ_TRY(launch_prep(pa, st));
     }
 
@@ -705,7 +707,7 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
                             max_holes_, record_ext_ ? ext_.as<uint4>() : nullptr, grid, st));
     SPJ_TRY(launch_nphj(d_insts, reinterpret_cast<const NphjJob *>(dd + o_nphj), (uint32_t)h_nphj_.size(),
                         nphj_.as<uint64_t>(), nphj_words, (uint32_t)nphj_bb, (uint32_t)nphj_pb, out_lo, out_fill,
-                        err, st));
+                        err, false, st));
     if (timed) SPJ_TRY(hipEventRecord(ev_[6], st));
     specs_n_ = n;
     h_ns_.resize(n);

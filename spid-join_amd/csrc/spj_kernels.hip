@@ -287,12 +287,14 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a)
 {
     if (blockIdx.y == 0) {
         const uint64_t n0 = a.nctrl, n1 = n0 + a.nheavy, n2 = n1 + a.ncand, n3 = n2 + a.nzero, n4 = n3 + a.ntiles;
-        for (uint64_t w = (uint64_t)blockIdx.x * 256u + threadIdx.x; w < n4; w += (uint64_t)gridDim.x * 256u) {
+        const uint64_t n5 = n4 + a.nnphj;
+        for (uint64_t w = (uint64_t)blockIdx.x * 256u + threadIdx.x; w < n5; w += (uint64_t)gridDim.x * 256u) {
             if (w < n0) a.ctrl[w] = 0;
             else if (w < n1) a.heavy[w - n0] = 0;
             else if (w < n2) a.cand[w - n1] = 0;
             else if (w < n3) a.counts[w - n2] = 0;
-            else expand_tile(a.runs, a.nruns, a.cells, a.tiles, (uint32_t)(w - n3));
+            else if (w < n4) expand_tile(a.runs, a.nruns, a.cells, a.tiles, (uint32_t)(w - n3));
+            else a.nphj[w - n4] = 0ull;
         }
     } else {
         for (uint32_t c = blockIdx.x; c < a.ncells; c += gridDim.x) {
@@ -2364,10 +2366,10 @@ uint32_t cand_words() { return 2 * kCandSlots + 4; }
 
 hipError_t launch_prep(const PrepArgs &a, hipStream_t st)
 {
-    const uint64_t flat = a.nctrl + a.nheavy + a.ncand + a.nzero + a.ntiles;
+    const uint64_t flat = a.nctrl + a.nheavy + a.ncand + a.nzero + a.ntiles + a.nnphj;
     uint64_t bx = (flat + 255) / 256;
     if (bx < a.ncells) bx = a.ncells;
-    if (bx > 4096) bx = 4096;
+    if (bx > 65536) bx = 65536;   // (large no-partition tables: 16M threads, as their own fill had)
     if (!bx) return hipSuccess;
     hipLaunchKernelGGL(k_prep, dim3((uint32_t)bx, a.ncells ? 2u : 1u), dim3(256), 0, st, a);
     return hipGetLastError();
@@ -2397,14 +2399,14 @@ hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *c
 
 hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njobs, uint64_t *tables,
                        uint64_t table_words, uint32_t build_blocks, uint32_t probe_blocks, uint32_t *out_lo,
-                       uint32_t *out_fill, uint32_t *err, hipStream_t st)
+                       uint32_t *out_fill, uint32_t *err, bool clear, hipStream_t st)
 {
     if (!njobs) return hipSuccess;
     // a fill kernel rather than a memset node, so that the sequence stays
-    // capturable into a graph
+    // capturable into a graph (the batch path clears the tables in k_prep)
     auto *t = reinterpret_cast<unsigned long long *>(tables);
     const uint64_t fb = std::min<uint64_t>((table_words + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_nphj_clear, dim3((uint32_t)fb), dim3(256), 0, st, t, table_words);
+    if (clear) hipLaunchKernelGGL(k_nphj_clear, dim3((uint32_t)fb), dim3(256), 0, st, t, table_words);
     if (build_blocks)
         hipLaunchKernelGGL(k_nphj_build, dim3(build_blocks), dim3(256), 0, st, insts, jobs, njobs, t, err);
     if (probe_blocks)
@@ -2513,7 +2515,7 @@ hipError_t run_nphj_noalloc(const uint2 *R, uint32_t nR, const uint2 *S, uint32_
     const uint32_t bb = (nR + kNphjBuildTuples - 1) / kNphjBuildTuples;
     const uint32_t pb = (uint32_t)(((uint64_t)nS + kNphjProbeTuples - 1) / kNphjProbeTuples);
     e = launch_nphj(d_inst, d_job, 1, reinterpret_cast<uint64_t *>(head + kNaHead), uint64_t(1) << j.log2cap, bb, pb,
-                    ctrl, ctrl + 1, ctrl + 2, st);
+                    ctrl, ctrl + 1, ctrl + 2, true, st);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(ctrl_out, ctrl, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;

@@ -54,7 +54,7 @@ hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *c
 // match at out_lo[inst] (out_fill advanced by the same count).
 hipError_t launch_nphj(const InstDesc *insts, const NphjJob *jobs, uint32_t njobs, uint64_t *tables,
                        uint64_t table_words, uint32_t build_blocks, uint32_t probe_blocks, uint32_t *out_lo,
-                       uint32_t *out_fill, uint32_t *err, hipStream_t st);
+                       uint32_t *out_fill, uint32_t *err, bool clear, hipStream_t st);
 // The no-allocation join (no partition pass) of one instance of at most
 // kNoAllocMaxR R tuples: descriptors, control words and table live in a
 // static device array (one per device, held until the process ends), so
