@@ -1208,6 +1208,10 @@ constexpr uint32_t kSubFill = 16384;        // mean sub-table fill when R_p need
 constexpr uint32_t kResplits = 2;           // sub-table overflows re-split (2Q) before sort-merge
 constexpr uint32_t kRangeSortMax = 64;      // sort-merge: longest key-range bucket sorted in place
 constexpr uint32_t kScan = 4;               // bucket entries read unconditionally per probe
+constexpr uint32_t kBigList = 3584;         // sort-merge: key-range buckets of 3+ entries listed (u16, LDS)
+// byte offset of that list: after the table ((kTableCap + kScan) pairs) and
+// its bucket ends (7 + 1 + kBuckets halves, the ends 16-byte aligned)
+constexpr size_t kBigListOff = ((2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 8) * 2) + 15) / 16 * 16;
 constexpr uint32_t kSortR = 16384;          // sort-merge: R run sorted in LDS (u64 words)
 constexpr uint32_t kSortTail = 1024;        // sort-merge: a second, short run in the same pass
 constexpr uint32_t kSortPad = kSortR + kSortR / 16;   // the register sort's padded layout
@@ -1224,6 +1228,7 @@ struct JoinShared {
     uint32_t kmin;   // sort-merge: the smallest and largest R key of the partition
     uint32_t kmax;
     uint32_t big;    // sort-merge: some key-range bucket is too long to sort in place
+    uint32_t nbig;   // sort-merge: key-range buckets of three or more entries listed
 };
 
 // Append one pair per lane whose `hit` is set.  Positions come from a
@@ -1628,6 +1633,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             sh.kmin = 0xFFFFFFFFu;
                             sh.kmax = 0u;
                             sh.big = 0u;
+                            sh.nbig = 0u;
                         }
                         __syncthreads();
                         // R_p fits one table (Q == 1): hold it in registers (<= RPT
@@ -1861,19 +1867,15 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                             // adjacent: duplicates (mpsm_join_sort_probe.c sorts each
                             // range bucket, 152-236; a2a_srj_build.c:265-271).
                             bool dup = false, big = false;
-                            uint32_t b0 = threadIdx.x;   // opaque: addresses formed here
-                            asm volatile("" : "+v"(b0));
-                            for (uint32_t bb = b0; bb < nb; bb += NT) {
-                                const uint32_t st = bend[bb], n = bend[bb + 1] - st;
-                                if (n <= 1) continue;
+                            // one bucket of n >= 3 entries from st: eight or fewer by
+                            // a network on (key, rid) as 64-bit words (slots past n
+                            // sort last; Batcher's odd-even merge, 19 comparators),
+                            // up to kRangeSortMax by insertion, longer ones send the
+                            // item to the run sort.  Slots past the bucket may lie
+                            // past the table's slack, in the bucket ends: read, never
+                            // used.
+                            auto sort_bucket = [&](uint32_t st, uint32_t n) {
                                 if (n <= 8) {
-                                    // (key, rid) as 64-bit words, slots past n sort
-                                    // last; Batcher's odd-even merge network of 19
-                                    // comparators.  A bucket up to 8 entries long
-                                    // (1 in 4000 are longer at 2 entries per bucket
-                                    // on average) never waits on dependent LDS reads.
-                                    // Slots past the bucket may lie past the table's
-                                    // slack, in the bucket ends: read, never used.
                                     uint64_t e[8];
 #pragma unroll
                                     for (uint32_t u = 0; u < 8; u++) {
@@ -1910,6 +1912,48 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                 } else {
                                     big = true;
                                 }
+                            };
+                            // Pass 1: a bucket of two entries is sorted by one
+                            // comparison; the longer ones are listed (one LDS atomic
+                            // per wave).  Pass 2 deals the listed buckets to the
+                            // threads evenly: with about two entries per bucket, a
+                            // third of them need the network, and every lane ran it
+                            // for each of its buckets before (a wave waits for its
+                            // longest bucket).  A bucket the list has no room for is
+                            // sorted in pass 1.
+                            uint16_t *blist = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(lds) + kBigListOff);
+                            uint32_t b0 = threadIdx.x;   // opaque: addresses formed here
+                            asm volatile("" : "+v"(b0));
+                            for (uint32_t bb = b0; bb < nb; bb += NT) {
+                                const uint32_t st = bend[bb], n = bend[bb + 1] - st;
+                                if (n == 2) {
+                                    const uint2 x = tkv[st], y = tkv[st + 1];
+                                    const bool sw = x.x > y.x || (x.x == y.x && x.y > y.y);
+                                    if (sw) {
+                                        tkv[st] = y;
+                                        tkv[st + 1] = x;
+                                    }
+                                    dup |= x.x == y.x;
+                                }
+                                const bool want = n >= 3;
+                                const uint64_t m = __ballot(want);
+                                if (m) {
+                                    uint32_t wb = 0;
+                                    if (lane_id() == 0) wb = atomicAdd(&sh.nbig, (uint32_t)__popcll(m));
+                                    wb = __shfl(wb, 0, kWave);
+                                    if (want) {
+                                        const uint32_t pos = wb + mbcnt(m);
+                                        if (pos < kBigList) blist[pos] = (uint16_t)bb;
+                                        else sort_bucket(st, n);
+                                    }
+                                }
+                            }
+                            __syncthreads();   // the list is complete
+                            const uint32_t nbig = min(sh.nbig, kBigList);
+                            for (uint32_t i = b0; i < nbig; i += NT) {
+                                const uint32_t bb = blist[i];
+                                const uint32_t st = bend[bb];
+                                sort_bucket(st, (uint32_t)bend[bb + 1] - st);
                             }
                             if (dup) atomicOr(erri, kErrDuplicateKey);
                             if (big) sh.big = 1u;
@@ -2273,7 +2317,8 @@ size_t lds_scatter(uint32_t max_bits, int ipt)
 
 size_t lds_join()
 {
-    size_t hash = 2 * ((size_t)kTableCap + kScan) * 4 + (kBuckets + 8) * 2;   // ends 16-byte aligned: 7 + 1 + kBuckets halves
+    // the hash table and its bucket ends, then the sort-merge list of long buckets
+    size_t hash = kBigListOff + (size_t)kBigList * 2;
     size_t smj = ((size_t)kSortPad + kSortTail) * 8;
     size_t nl = 2 * kNLChunk * 4;
     size_t m = hash > smj ? hash : smj;
