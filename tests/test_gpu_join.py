@@ -671,6 +671,61 @@ def test_sort_merge_key_range_buckets(gpu, orc, layout):
         assert e.value.code == -4
 
 
+def range_buckets(keys, nb=8192):
+    """Key-range bucket of each R key of a sort-merge partition: (key - kmin)
+    * floor(nb 2^32 / (kmax - kmin + 1)) >> 32.  Used only to check that an
+    input reaches the bucket counts a test is about."""
+    k = keys.astype(np.uint64)
+    lo, hi = int(k.min()), int(k.max())
+    scale = np.uint64(min((nb << 32) // (hi - lo + 1), 0xFFFFFFFF))
+    return ((k - np.uint64(lo)) * scale) >> np.uint64(32)
+
+
+def test_sort_merge_many_long_buckets(gpu, orc):
+    """One sort-merge partition (radix bits 0, 15K R tuples, one table)
+    whose keys come in clusters of 3 to 8 consecutive keys, one cluster per
+    key-range bucket, next to buckets of one and two keys: more buckets of
+    three or more entries than the list that deals them to the threads
+    holds (3584), so some are sorted where they are found.  Full-range R
+    rids; a duplicate inside a long bucket is SPJ_EDUPKEY wherever it is
+    sorted."""
+    sj = gpu
+    rng = np.random.default_rng(77)
+    sizes = np.concatenate([np.full(1850, 3), np.full(1850, 4), 5 + np.arange(120) % 4,
+                            np.full(400, 2), np.full(400, 1)])
+    sizes = sizes[rng.permutation(len(sizes))]
+    # cluster c starts at 2000 c + 10: bucket width ~1000 keys, so a cluster
+    # of at most 8 consecutive keys shares a bucket with no other cluster
+    keys = np.concatenate([np.arange(s, dtype=np.uint32) + np.uint32(2000 * c + 10) for c, s in enumerate(sizes)])
+    n = len(keys)
+    assert n <= 17400
+    counts = np.bincount(range_buckets(keys).astype(np.int64))
+    assert (counts >= 3).sum() > 3584 and counts.max() <= 8
+    R = np.zeros(n, dtype=sj.TUPLE)
+    perm = rng.permutation(n)
+    R["key"] = keys[perm]
+    R["rid"] = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    S = np.zeros(4 * n, dtype=sj.TUPLE)
+    S["key"] = R["key"][rng.integers(0, n, len(S))]
+    S["key"][::9] += np.uint32(1000)   # between clusters: unmatched
+    S["rid"] = np.arange(len(S), dtype=np.uint32)
+    cfg = sj.config(algorithm=2, radix_bits=0)
+    print("case smj long buckets", flush=True)
+    pairs, _ = sj.join(R, S, cfg)
+    check_equal(sj, orc, R, S, pairs)
+    # duplicates in long buckets at the start, middle and end of the range
+    for c in (3, len(sizes) // 2, len(sizes) - 3):
+        if sizes[c] < 3:
+            c += 1 if sizes[c + 1] >= 3 else 2
+        first = int(sizes[:c].sum())
+        R2 = R.copy()
+        i0, i1 = np.flatnonzero(perm == first)[0], np.flatnonzero(perm == first + 2)[0]
+        R2["key"][i1] = R2["key"][i0]
+        with pytest.raises(sj.SpjError) as e:
+            sj.join(R2, S, cfg)
+        assert e.value.code == -4, c
+
+
 @pytest.mark.parametrize("ranks,alg", [(16, 0), (4, 2), (8, 3)])
 def test_rank_padded_result_layout(gpu, orc, ranks, alg):
     """FinishJoin in the reference's gathered layout: one buffer per rank of
