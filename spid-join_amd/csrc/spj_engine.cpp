@@ -702,9 +702,11 @@ hipError_t Context::enqueue_plan(const std::vector<InstSpec> &specs_in, PhaseTim
     // items, k_join 59.5 us with 256 workgroups, 109.8 us with 7)
     const uint32_t grid = cus;
     if (record_ext_) SPJ_TRY(ext_.ensure(items_bound * sizeof(uint4)));
+    bool range = false;   // a sort-merge instance: the join kernel with key-range buckets
+    for (const InstDesc &d : h_insts_) range |= d.alg == 2;
     if (n_joined)
         SPJ_TRY(launch_join(d_insts, items_.as<WorkItem>(), ctrl, out_lo, out_fill, err, holes_.as<uint4>(),
-                            max_holes_, record_ext_ ? ext_.as<uint4>() : nullptr, grid, st));
+                            max_holes_, record_ext_ ? ext_.as<uint4>() : nullptr, grid, range, st));
     SPJ_TRY(launch_nphj(d_insts, reinterpret_cast<const NphjJob *>(dd + o_nphj), (uint32_t)h_nphj_.size(),
                         nphj_.as<uint64_t>(), nphj_words, (uint32_t)nphj_bb, (uint32_t)nphj_pb, out_lo, out_fill,
                         err, false, st));

@@ -1359,6 +1359,11 @@ struct ItemRegs {
 };
 static_assert(kItemWords == 40 && kItemWords <= kWave, "ItemRegs holds 40 words, one wave copies them");
 
+// kRange: with the sort-merge of partitions that fit one table (key-range
+// buckets).  The instantiation without it serves batches with no sort-merge
+// instance; its registers are allocated for the hash and nested-loop paths
+// alone (an alg 2 item there takes the run sort, so it stays correct).
+template <bool kRange>
 __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ insts,
                                                   const WorkItem *__restrict__ items, uint32_t *__restrict__ ctrl,
                                                   uint32_t *__restrict__ out_lo, uint32_t *__restrict__ out_fill,
@@ -1447,7 +1452,7 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         }
                     }
                 }
-            } else if (alg == 2 && (run_sort || nr > kTableCap)) {
+            } else if (alg == 2 && (!kRange || run_sort || nr > kTableCap)) {
                 // ---------------- sort-merge in runs (mpsm_join_sort_probe.c),
                 // for a partition larger than one table or with keys too
                 // clustered for the key-range buckets below: the R
@@ -1913,47 +1918,92 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                                     big = true;
                                 }
                             };
-                            // Pass 1: a bucket of two entries is sorted by one
+                            // Pass 1: thread t takes buckets 8t .. 8t + 7 (their ends
+                            // one 128-bit word, the first start the word before's
+                            // last end).  A bucket of two entries is sorted by one
                             // comparison; the longer ones are listed (one LDS atomic
                             // per wave).  Pass 2 deals the listed buckets to the
                             // threads evenly: with about two entries per bucket, a
                             // third of them need the network, and every lane ran it
                             // for each of its buckets before (a wave waits for its
-                            // longest bucket).  A bucket the list has no room for is
-                            // sorted in pass 1.
+                            // longest bucket).  If the list overflows (more than
+                            // kBigList such buckets: keys in clusters), pass 3 sorts
+                            // every thread's own long buckets again -- a sorted
+                            // bucket sorts to itself, with the same duplicates.
                             uint16_t *blist = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(lds) + kBigListOff);
                             uint32_t b0 = threadIdx.x;   // opaque: addresses formed here
                             asm volatile("" : "+v"(b0));
-                            for (uint32_t bb = b0; bb < nb; bb += NT) {
-                                const uint32_t st = bend[bb], n = bend[bb + 1] - st;
-                                if (n == 2) {
-                                    const uint2 x = tkv[st], y = tkv[st + 1];
-                                    const bool sw = x.x > y.x || (x.x == y.x && x.y > y.y);
-                                    if (sw) {
-                                        tkv[st] = y;
-                                        tkv[st + 1] = x;
-                                    }
-                                    dup |= x.x == y.x;
+                            {
+                                uint32_t be[9];   // bucket 8t + k is [be[k], be[k + 1])
+#pragma unroll
+                                for (uint32_t k = 0; k < 9; k++) be[k] = 0;
+                                if (b0 < (nb >> 3)) {   // nb >= 64, a multiple of 8
+                                    const uint4 c = reinterpret_cast<const uint4 *>(ends)[b0];
+                                    be[0] = bend[8 * b0];
+                                    be[1] = c.x & 0xFFFFu;
+                                    be[2] = c.x >> 16;
+                                    be[3] = c.y & 0xFFFFu;
+                                    be[4] = c.y >> 16;
+                                    be[5] = c.z & 0xFFFFu;
+                                    be[6] = c.z >> 16;
+                                    be[7] = c.w & 0xFFFFu;
+                                    be[8] = c.w >> 16;
                                 }
-                                const bool want = n >= 3;
-                                const uint64_t m = __ballot(want);
-                                if (m) {
+                                // the pairs' entries read four buckets at a time, one wait each
+#pragma unroll
+                                for (uint32_t h = 0; h < 8; h += 4) {
+                                    uint2 xa[4], ya[4];
+#pragma unroll
+                                    for (uint32_t k = 0; k < 4; k++) {
+                                        const uint32_t st = be[h + k];
+                                        if (be[h + k + 1] - st == 2) {
+                                            xa[k] = tkv[st];
+                                            ya[k] = tkv[st + 1];
+                                        }
+                                    }
+#pragma unroll
+                                    for (uint32_t k = 0; k < 4; k++) {
+                                        const uint32_t st = be[h + k];
+                                        if (be[h + k + 1] - st != 2) continue;
+                                        const uint2 x = xa[k], y = ya[k];
+                                        if (x.x > y.x || (x.x == y.x && x.y > y.y)) {
+                                            tkv[st] = y;
+                                            tkv[st + 1] = x;
+                                        }
+                                        dup |= x.x == y.x;
+                                    }
+                                }
+                                uint32_t c3 = 0;
+#pragma unroll
+                                for (uint32_t k = 0; k < 8; k++) c3 += be[k + 1] - be[k] >= 3 ? 1u : 0u;
+                                const uint32_t incl = wave_incl_sum(c3);
+                                const uint32_t wtot = __builtin_amdgcn_readlane(incl, kWave - 1);
+                                if (wtot) {   // uniform
                                     uint32_t wb = 0;
-                                    if (lane_id() == 0) wb = atomicAdd(&sh.nbig, (uint32_t)__popcll(m));
-                                    wb = __shfl(wb, 0, kWave);
-                                    if (want) {
-                                        const uint32_t pos = wb + mbcnt(m);
-                                        if (pos < kBigList) blist[pos] = (uint16_t)bb;
-                                        else sort_bucket(st, n);
+                                    if (lane_id() == kWave - 1) wb = atomicAdd(&sh.nbig, wtot);
+                                    uint32_t pos = __builtin_amdgcn_readlane(wb, kWave - 1) + incl - c3;
+#pragma unroll
+                                    for (uint32_t k = 0; k < 8; k++) {
+                                        if (be[k + 1] - be[k] < 3) continue;
+                                        if (pos < kBigList) blist[pos] = (uint16_t)(8 * b0 + k);
+                                        pos++;
                                     }
                                 }
                             }
                             __syncthreads();   // the list is complete
-                            const uint32_t nbig = min(sh.nbig, kBigList);
-                            for (uint32_t i = b0; i < nbig; i += NT) {
+                            const uint32_t nbig = sh.nbig;
+                            for (uint32_t i = b0; i < min(nbig, kBigList); i += NT) {
                                 const uint32_t bb = blist[i];
                                 const uint32_t st = bend[bb];
                                 sort_bucket(st, (uint32_t)bend[bb + 1] - st);
+                            }
+                            if (nbig > kBigList) {   // uniform
+                                __syncthreads();   // pass 2 is done with every bucket
+                                if (b0 < (nb >> 3))
+                                    for (uint32_t bb = 8 * b0; bb < 8 * b0 + 8; bb++) {
+                                        const uint32_t st = bend[bb], n = bend[bb + 1] - st;
+                                        if (n >= 3) sort_bucket(st, n);
+                                    }
                             }
                             if (dup) atomicOr(erri, kErrDuplicateKey);
                             if (big) sh.big = 1u;
@@ -2098,8 +2148,12 @@ __global__ __launch_bounds__(kJoinNT) void k_join(const InstDesc *__restrict__ i
                         q++;
                     }
                 };
-                if (alg == 2) table_join(std::true_type{});
-                else table_join(std::false_type{});
+                if constexpr (kRange) {
+                    if (alg == 2) table_join(std::true_type{});
+                    else table_join(std::false_type{});
+                } else {
+                    table_join(std::false_type{});
+                }
                 if (to_smj) {   // sh.fill was reset above (hash), or nothing was written (sort-merge)
                     alg = 2;
                     run_sort = true;
@@ -2432,13 +2486,18 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
 
 hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
-                       uint4 *ext, uint32_t grid, hipStream_t st)
+                       uint4 *ext, uint32_t grid, bool range, hipStream_t st)
 {
     size_t l = lds_join();
-    hipError_t e = set_lds((const void *)k_join, l);
+    const void *k = range ? (const void *)k_join<true> : (const void *)k_join<false>;
+    hipError_t e = set_lds(k, l);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
-                       holes, max_holes, ext);
+    if (range)
+        hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
+                           holes, max_holes, ext);
+    else
+        hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinNT), l, st, insts, items, ctrl, out_lo, out_fill, err,
+                           holes, max_holes, ext);
     return hipGetLastError();
 }
 

@@ -45,10 +45,11 @@ hipError_t launch_make_items(const InstDesc *insts, uint32_t ninst, const uint32
                              const uint32_t *counts, uint32_t smax, WorkItem *items, uint32_t *nitems,
                              uint32_t *spart, uint32_t *rpart, hipStream_t st);
 // ext (optional, one entry per work item): (instance, partition, first
-// output position, pairs written) of every item.
+// output position, pairs written) of every item.  range: some instance is a
+// sort-merge join (the kernel with key-range buckets is launched).
 hipError_t launch_join(const InstDesc *insts, const WorkItem *items, uint32_t *ctrl,
                        uint32_t *out_lo, uint32_t *out_fill, uint32_t *err, uint4 *holes, uint32_t max_holes,
-                       uint4 *ext, uint32_t grid, hipStream_t st);
+                       uint4 *ext, uint32_t grid, bool range, hipStream_t st);
 // No-partition hash join of every job's instance: zero the tables, insert
 // R (a duplicate key flags kErrDuplicateKey), then probe S and append each
 // match at out_lo[inst] (out_fill advanced by the same count).
