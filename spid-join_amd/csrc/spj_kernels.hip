@@ -853,10 +853,17 @@ __global__ __launch_bounds__(1024) void k_heavy_sample(const InstDesc *__restric
         __syncthreads();
         uint32_t *cand_keys = cand_base + job.cand_off, *cand_cnt = cand_keys + kCandSlots;
         uint32_t *found = cand_cnt + kCandSlots;
-        for (uint32_t i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
-            const uint32_t cnt = sc[i];
+        // the counts are read all at once (one wait), then the few keys seen
+        // kLocalMin times go to the accumulator
+        constexpr uint32_t SPT = kSampleSlots / 1024;
+        uint32_t cnts[SPT];
+#pragma unroll
+        for (uint32_t u = 0; u < SPT; u++) cnts[u] = sc[u * 1024 + threadIdx.x];
+#pragma unroll
+        for (uint32_t u = 0; u < SPT; u++) {
+            const uint32_t cnt = cnts[u];
             if (cnt < kLocalMin) continue;
-            const uint32_t key = sk[i];
+            const uint32_t key = sk[u * 1024 + threadIdx.x];
             uint32_t s = (key * 0x9E3779B1u) >> (32 - 11);
             bool placed = false;
             for (uint32_t step = 0; step < kAccSlots; step++) {
@@ -1034,13 +1041,26 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     __syncthreads();
     const uint32_t nsel = min(n, kSel);
     if (nsel == 0) return;   // the table and count stay zero
-    // hottest first: ascending sort, then placement from the end in rounds
-    // of kWaveKeys keys, so that a key left out for two full buckets is
+    // hottest first: placement in rounds of kWaveKeys keys in descending
+    // (count, key) order, so that a key left out for two full buckets is
     // one of the coldest selected (a hot key left ordinary can overflow a
-    // cell of the partition layout)
+    // cell of the partition layout).  Up to 1024 candidates (the usual
+    // case) each thread ranks one by counting the larger ones, read as LDS
+    // broadcasts, and stores it at its rank; more are sorted ascending by a
+    // bitonic network (55 barrier stages already at 1024 elements).
     uint32_t m = 1;
     while (m < nsel) m <<= 1;
-    bitonic_sort_lds(ord, m);   // ord[nsel, m) are zeros: the keys end up in ord[m - nsel, m)
+    const bool ranked = nsel <= 1024;
+    if (ranked) {
+        const uint64_t mine = threadIdx.x < nsel ? ord[threadIdx.x] : 0ull;
+        uint32_t rk = 0;
+        for (uint32_t j = 0; j < nsel; j++) rk += ord[j] > mine ? 1u : 0u;
+        __syncthreads();   // every read of ord is done
+        if (threadIdx.x < nsel) ord[rk] = mine;   // (count << 32 | key) is unique: the ranks are a permutation
+        __syncthreads();
+    } else {
+        bitonic_sort_lds(ord, m);   // ord[nsel, m) are zeros: the keys end up in ord[m - nsel, m)
+    }
     const uint32_t nkeys = min(nsel, kHeavyMaxKeys);
     // 2-choice placement, one key per thread per round: a slot of the
     // emptier of the key's two 4-slot buckets (as seen now), else of the
@@ -1048,7 +1068,8 @@ __global__ __launch_bounds__(1024) void k_heavy_select(const InstDesc *__restric
     // unique, so each sits in at most one slot.
     for (uint32_t r0 = 0; r0 < nkeys; r0 += kWaveKeys) {
         if (threadIdx.x < kWaveKeys && r0 + threadIdx.x < nkeys) {
-            const uint32_t key = (uint32_t)ord[m - 1 - (r0 + threadIdx.x)];
+            const uint32_t r = r0 + threadIdx.x;
+            const uint32_t key = (uint32_t)ord[ranked ? r : m - 1 - r];
             const uint32_t h = sfh(key), b1 = heavy_b1(h), b2 = heavy_b2(h);
             const uint32_t first = bfill[b2] < bfill[b1] ? b2 : b1, second = first == b1 ? b2 : b1;
             uint32_t slot = atomicAdd(&bfill[first], 1u), b = first;
